@@ -1,0 +1,174 @@
+/*
+ * ticc_hip.h -- C ABI of the MI355X (gfx950) coupled-channel backend.
+ *
+ * One shared library, libticc_hip.so, built with hipcc for gfx950.  Every entry
+ * point takes plain pointers and sizes; "device" pointers are HIP device
+ * addresses, "host" pointers are ordinary CPU memory.  All work is enqueued on
+ * `stream` (a hipStream_t passed as void*; NULL means the legacy default
+ * stream) and is asynchronous unless stated otherwise.  Matrices are FP64,
+ * row-major, contiguous.
+ *
+ * Return value: TICC_OK (0) or one of the TICC_ERR_* codes; ticc_last_error()
+ * then returns a thread-local message.  The Python layer maps
+ *   TICC_ERR_VALUE -> ValueError, TICC_ERR_UNSUPPORTED -> NotImplementedError,
+ *   TICC_ERR_NONFINITE -> FloatingPointError, everything else -> RuntimeError,
+ * which mirrors the reference's validate / logger.error / raise convention.
+ *
+ * The reference (PyTICC) has no foreign-function boundary: its hot path is
+ * jax.numpy lowered by XLA.  Each entry point below replaces one of its
+ * internal operator seams; the reference file:line is cited per function.
+ */
+#ifndef TICC_HIP_H
+#define TICC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TICC_ABI_VERSION 1
+
+#define TICC_OK 0
+#define TICC_ERR_VALUE 1
+#define TICC_ERR_HIP 2
+#define TICC_ERR_NOMEM 3
+#define TICC_ERR_NONFINITE 4
+#define TICC_ERR_UNSUPPORTED 5
+
+/* Largest channel count of one propagated block. */
+#define TICC_MAX_CHANNELS 1024
+
+/* ABI version of the loaded library (TICC_ABI_VERSION). */
+int ticc_abi_version(void);
+
+/* Thread-local message describing the last non-zero status of this thread. */
+const char* ticc_last_error(void);
+
+/* Number of visible HIP devices (0 when no GPU is present).  Never fails. */
+int ticc_device_count(int* count);
+
+/* Make `device` current for the calling thread (hipSetDevice).
+ * Replaces resolve_device / jax.devices (src/pyticc/propagation/device.py:52-71). */
+int ticc_set_device(int device);
+
+/* Free every library-owned workspace on the current device (synchronises it). */
+int ticc_release_workspace(void);
+
+/*
+ * W(R) contraction + radial-matrix assembly, kernel A.
+ * Replaces contract_device / _contract_block_device
+ * (src/pyticc/matrix/interaction/common.py:66-73, 153-211) and the device W
+ * assembly W = U/R^2 + 2 mu V + 2 mu diag(E_int) (src/pyticc/propagation/runner.py:265-271).
+ *
+ *   n_channel        N
+ *   n_ang            angular grid points per R (prod of the PES grid shape)
+ *   n_block          number of helicity (K) blocks
+ *   block_offsets    host int[n_block+1]: rows of block k are basis rows
+ *                    [block_offsets[k], block_offsets[k+1])
+ *   block_channels   device int[N]: channel index of each basis row
+ *   B_real, B_imag   device (N, n_ang) weighted basis rows grouped by block;
+ *                    B_imag may be NULL (atom-diatom)
+ *   normalization    1/pi for AB+CD, 1 for A+BC
+ *   n_R, R           device double[n_R] radial points
+ *   V                device (n_R, n_ang) raw PES values at those points
+ *   U, E_int         device (N, N) centrifugal matrix, device double[N] thresholds
+ *   W                device (n_R, N, N) output, fully written
+ */
+int ticc_contract_w(int n_channel, int n_ang, int n_block, const int* block_offsets, const int* block_channels,
+                    const double* B_real, const double* B_imag, double normalization, int n_R, const double* R,
+                    const double* V, const double* U, const double* E_int, double reduced_mass, double* W,
+                    void* stream);
+
+/*
+ * Initial log-derivative for the inelastic boundary condition,
+ * Y_e = diag(sqrt|W0_ii - 2 mu E_e|).
+ * Replaces initialize_logD_inelastic (src/pyticc/propagation/logd.py:48-65) as
+ * called at src/pyticc/propagation/runner.py:284-291.
+ *   W0 device (N, N) energy-independent W_base at the first radial point;
+ *   energies host double[n_energy]; Y device (n_energy, N, N) output.
+ */
+int ticc_ldmd_init(int n_energy, int n_channel, const double* W0, const double* energies, double reduced_mass,
+                   double* Y, void* stream);
+
+/*
+ * LDMD sweep of an energy batch over n_sector sectors, kernel B.
+ * Replaces propagate_logD / _propagate_logD_jax
+ * (src/pyticc/propagation/logd.py:288-330, 340-398), including the
+ * energy-independent corrections (_correction_matrices, logd.py:143-164) and the
+ * per-energy two-half-step update (_propagate_with_corrections, logd.py:168-194).
+ *
+ *   energies       host double[n_energy]
+ *   half_steps     host double[n_sector], each > 0
+ *   W_start, W_mid, W_end
+ *                  device pointers to the energy-independent W_base of sector
+ *                  0; sector s uses pointer + s * w_sector_stride (elements).
+ *                  (W_points of 2 n_sector + 1 radial points: W_start = W,
+ *                  W_mid = W + N*N, W_end = W + 2*N*N, stride 2*N*N.)
+ *   Y              device (n_energy, N, N), in: Y at the first sector start,
+ *                  out: Y at the last sector end.
+ * A non-finite result returns TICC_ERR_NONFINITE (the reference silently
+ * propagates NaN).  Synchronises `stream` once at the end for that check.
+ */
+int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, double reduced_mass, int n_sector,
+                        const double* half_steps, const double* W_start, const double* W_mid, const double* W_end,
+                        long long w_sector_stride, double* Y, void* stream);
+
+/*
+ * Asymptotic matching, kernel C: Y_SF = B^T Y B, then per energy the
+ * closed-channel elimination and K -> S.
+ * Replaces transform_logD_BF_to_SF (src/pyticc/match/asymptotic.py:82-110) and
+ * get_Smat / _open_reaction_matrix / _reference_functions
+ * (src/pyticc/match/smatrix.py:10-223), with the Riccati-Bessel and K_nu
+ * log-derivative references of src/pyticc/match/bessel.py:7-45, 90-122
+ * evaluated on the device.
+ *
+ *   Y        device (n_energy, N, N) BF log derivatives
+ *   B        device (N, N) BF->SF orthogonal transform (NULL: identity)
+ *   energies host double[n_energy]
+ *   E_int, L host double[N]
+ *   S_real, S_imag  device (n_energy, N, N) outputs; for energy e the leading
+ *            n_open(e) x n_open(e) block holds S in flatnonzero(E_int < E)
+ *            order, the rest is zero.
+ *   n_open   host int[n_energy] output.
+ * Raises TICC_ERR_VALUE if an energy equals a threshold or L < 0.
+ */
+int ticc_get_smat(int n_energy, int n_channel, const double* Y, const double* B, double Rmatch,
+                  const double* energies, double reduced_mass, const double* E_int, const double* L, double* S_real,
+                  double* S_imag, int* n_open, void* stream);
+
+/*
+ * Device reference functions (for tests): for each of n points,
+ * out[4*i .. 4*i+3] = riccati j, n, j', n' of order ell[i] at x[i]
+ * (src/pyticc/match/bessel.py:7-45);  kout[i] = K'_nu/K_nu at (nu[i], x[i])
+ * (bessel.py:90-122).  Either output may be NULL.  All pointers device.
+ */
+int ticc_bessel_eval(int n, const double* ell, const double* x, double* out, const double* nu, double* kout,
+                     void* stream);
+
+/*
+ * Batched in-place inverse of n_batch matrices (n_batch, N, N), device, by
+ * blocked Gauss-Jordan with partial pivoting (the arithmetic of
+ * jnp.linalg.solve(A, I), src/pyticc/propagation/logd.py:162).
+ */
+int ticc_invert_batched(int n_batch, int n_channel, double* A, void* stream);
+
+/* Batched C = alpha A B + beta C for (m, k) x (k, n) row-major device matrices
+ * with batch strides (elements); stride 0 broadcasts.  (Test and utility entry.) */
+int ticc_dgemm_batched(int m, int n, int k, double alpha, const double* A, long long strideA, const double* B,
+                       long long strideB, double beta, double* C, long long strideC, int batch, void* stream);
+
+/*
+ * Instrumentation (no counterpart in the reference): live HIP-event timing of
+ * the rank-NB Gauss-Jordan update kernel, the sweep's dominant kernel.  Between
+ * begin and end, every sample_every-th batched inversion has each of its
+ * update launches bracketed by an event pair (at most max_samples pairs).
+ * end synchronises the device and returns the summed kernel time and the
+ * number of timed launches.
+ */
+int ticc_profile_update_begin(int sample_every, int max_samples);
+int ticc_profile_update_end(double* total_ms, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TICC_HIP_H */

@@ -1,0 +1,185 @@
+// extern "C" entry points of libticc_hip.so: argument validation and dispatch.
+#include <cmath>
+
+#include "ticc_common.h"
+
+namespace ticc {
+const char* last_error();
+int release_workspace();
+int contract_w(int N, int n_ang, int n_block, const int* block_offsets, const int* block_channels, const double* Br,
+               const double* Bi, double norm, int nR, const double* R, const double* V, const double* U,
+               const double* Eint, double mu, double* W, hipStream_t stream);
+int ldmd_init(int nE, int N, const double* W0, const double* E_host, double mu, double* Y, hipStream_t stream);
+int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const double* hs_host, const double* Ws0,
+                   const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream);
+int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, const double* E_host, double mu,
+             const double* Eint_host, const double* L_host, double* S_re, double* S_im, int* n_open,
+             hipStream_t stream);
+int profile_begin(int every, int max_pairs);
+int profile_end(double* total_ms, int* count);
+int bessel_eval(int n, const double* ell, const double* x, double* out, const double* nu, double* kout,
+                hipStream_t stream);
+}  // namespace ticc
+
+using ticc::fail;
+
+namespace {
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+int check_channels(int n) {
+  if (n < 1) return fail(TICC_ERR_VALUE, "n_channel must be positive, got %d", n);
+  if (n > TICC_MAX_CHANNELS)
+    return fail(TICC_ERR_UNSUPPORTED, "n_channel=%d exceeds the supported maximum %d", n, TICC_MAX_CHANNELS);
+  return TICC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ticc_abi_version(void) { return TICC_ABI_VERSION; }
+
+const char* ticc_last_error(void) { return ticc::last_error(); }
+
+int ticc_device_count(int* count) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    n = 0;
+  }
+  if (count) *count = n;
+  return TICC_OK;
+}
+
+int ticc_set_device(int device) {
+  int n = 0;
+  ticc_device_count(&n);
+  if (device < 0 || device >= n)
+    return fail(TICC_ERR_VALUE, "requested HIP device %d, but %d device(s) are visible", device, n);
+  TICC_CHECK_HIP(hipSetDevice(device));
+  return TICC_OK;
+}
+
+int ticc_release_workspace(void) { return ticc::release_workspace(); }
+
+int ticc_contract_w(int n_channel, int n_ang, int n_block, const int* block_offsets, const int* block_channels,
+                    const double* B_real, const double* B_imag, double normalization, int n_R, const double* R,
+                    const double* V, const double* U, const double* E_int, double reduced_mass, double* W,
+                    void* stream) {
+  TICC_RETURN_IF(check_channels(n_channel));
+  if (n_ang < 1 || n_block < 0 || n_R < 0) return fail(TICC_ERR_VALUE, "invalid contraction sizes");
+  if (!block_offsets || (n_block > 0 && (!block_channels || !B_real)) || !U || !E_int || !W || (n_R > 0 && (!R || !V)))
+    return fail(TICC_ERR_VALUE, "contraction: null pointer argument");
+  if (block_offsets[0] != 0) return fail(TICC_ERR_VALUE, "block_offsets[0] must be 0");
+  for (int k = 0; k < n_block; ++k)
+    if (block_offsets[k + 1] < block_offsets[k]) return fail(TICC_ERR_VALUE, "block_offsets must be non-decreasing");
+  if (block_offsets[n_block] > n_channel) return fail(TICC_ERR_VALUE, "basis rows exceed n_channel");
+  if (!(reduced_mass > 0.0)) return fail(TICC_ERR_VALUE, "reduced_mass must be positive");
+  return ticc::contract_w(n_channel, n_ang, n_block, block_offsets, block_channels, B_real, B_imag, normalization, n_R,
+                          R, V, U, E_int, reduced_mass, W, as_stream(stream));
+}
+
+int ticc_ldmd_init(int n_energy, int n_channel, const double* W0, const double* energies, double reduced_mass,
+                   double* Y, void* stream) {
+  TICC_RETURN_IF(check_channels(n_channel));
+  if (n_energy < 1) return fail(TICC_ERR_VALUE, "At least one total energy is required");
+  if (!W0 || !energies || !Y) return fail(TICC_ERR_VALUE, "ldmd_init: null pointer argument");
+  return ticc::ldmd_init(n_energy, n_channel, W0, energies, reduced_mass, Y, as_stream(stream));
+}
+
+int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, double reduced_mass, int n_sector,
+                        const double* half_steps, const double* W_start, const double* W_mid, const double* W_end,
+                        long long w_sector_stride, double* Y, void* stream) {
+  TICC_RETURN_IF(check_channels(n_channel));
+  if (n_energy < 1) return fail(TICC_ERR_VALUE, "At least one total energy is required");
+  if (n_sector < 0) return fail(TICC_ERR_VALUE, "n_sector must be non-negative");
+  if (!energies || !Y || (n_sector > 0 && (!half_steps || !W_start || !W_mid || !W_end)))
+    return fail(TICC_ERR_VALUE, "ldmd_propagate: null pointer argument");
+  for (int s = 0; s < n_sector; ++s)
+    if (!(half_steps[s] > 0.0) || !std::isfinite(half_steps[s]))
+      return fail(TICC_ERR_VALUE, "radial_half_step must be positive, but got radial_half_step=%.17g", half_steps[s]);
+  for (int e = 0; e < n_energy; ++e)
+    if (!std::isfinite(energies[e])) return fail(TICC_ERR_VALUE, "Total energies must be finite");
+  if (n_sector == 0) return TICC_OK;
+  return ticc::ldmd_propagate(n_energy, n_channel, energies, reduced_mass, n_sector, half_steps, W_start, W_mid, W_end,
+                              w_sector_stride, Y, as_stream(stream));
+}
+
+int ticc_get_smat(int n_energy, int n_channel, const double* Y, const double* B, double Rmatch,
+                  const double* energies, double reduced_mass, const double* E_int, const double* L, double* S_real,
+                  double* S_imag, int* n_open, void* stream) {
+  TICC_RETURN_IF(check_channels(n_channel));
+  if (n_energy < 1) return fail(TICC_ERR_VALUE, "At least one total energy is required");
+  if (!Y || !energies || !E_int || !L || !S_real || !S_imag || !n_open)
+    return fail(TICC_ERR_VALUE, "get_smat: null pointer argument");
+  if (!(Rmatch > 0.0) || !(reduced_mass > 0.0))
+    return fail(TICC_ERR_VALUE, "Rmatch and reduced_mass must be positive, but got Rmatch=%.17g, reduced_mass=%.17g",
+                Rmatch, reduced_mass);
+  for (int i = 0; i < n_channel; ++i)
+    if (!(L[i] >= 0.0)) return fail(TICC_ERR_VALUE, "Orbital angular momenta L must be non-negative");
+  return ticc::get_smat(n_energy, n_channel, Y, B, Rmatch, energies, reduced_mass, E_int, L, S_real, S_imag, n_open,
+                        as_stream(stream));
+}
+
+int ticc_bessel_eval(int n, const double* ell, const double* x, double* out, const double* nu, double* kout,
+                     void* stream) {
+  if (n < 0) return fail(TICC_ERR_VALUE, "n must be non-negative");
+  if (out && (!ell || !x)) return fail(TICC_ERR_VALUE, "bessel_eval: null pointer argument");
+  if (kout && (!nu || !x)) return fail(TICC_ERR_VALUE, "bessel_eval: null pointer argument");
+  TICC_RETURN_IF(ticc::bessel_eval(n, ell, x, out, nu, kout, as_stream(stream)));
+  TICC_CHECK_HIP(hipStreamSynchronize(as_stream(stream)));
+  return TICC_OK;
+}
+
+int ticc_invert_batched(int n_batch, int n_channel, double* A, void* stream) {
+  TICC_RETURN_IF(check_channels(n_channel));
+  if (n_batch < 1 || !A) return fail(TICC_ERR_VALUE, "invert_batched: invalid arguments");
+  hipStream_t st = as_stream(stream);
+  const int Np = ticc::round_up(n_channel, ticc::GJ_NB);
+  const size_t n2 = (size_t)Np * Np;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o0 = take(sizeof(double) * n_batch * n2), o1 = take(sizeof(double) * n_batch * n2);
+  const size_t oP = take(sizeof(double) * n_batch * Np * ticc::GJ_NB);
+  const size_t oS = take(sizeof(int) * n_batch * Np), oV = take(sizeof(int) * n_batch * ticc::GJ_NB);
+  const size_t oM = take(sizeof(int) * n_batch * Np), oI = take(sizeof(int) * n_batch * Np);
+  void* base = nullptr;
+  TICC_RETURN_IF(ticc::workspace(ticc::WS_INVERT, off, &base));
+  char* w = static_cast<char*>(base);
+  double* A0 = reinterpret_cast<double*>(w + o0);
+  double* A1 = reinterpret_cast<double*>(w + o1);
+  TICC_RETURN_IF(ticc::pad_identity(A, (long long)n_channel * n_channel, n_channel, A0, Np, n_batch, st));
+  double* res = nullptr;
+  int* pinv = reinterpret_cast<int*>(w + oI);
+  TICC_RETURN_IF(ticc::gj_invert_raw(A0, A1, n_batch, Np, &res, reinterpret_cast<int*>(w + oM), pinv,
+                                     reinterpret_cast<double*>(w + oP), reinterpret_cast<int*>(w + oS),
+                                     reinterpret_cast<int*>(w + oV), st));
+  double* other = (res == A0) ? A1 : A0;
+  TICC_RETURN_IF(ticc::gj_unpermute(res, other, n_batch, Np, pinv, st));
+  TICC_RETURN_IF(ticc::unpad(other, Np, A, (long long)n_channel * n_channel, n_channel, n_batch, st));
+  TICC_CHECK_HIP(hipStreamSynchronize(st));
+  return TICC_OK;
+}
+
+int ticc_dgemm_batched(int m, int n, int k, double alpha, const double* A, long long strideA, const double* B,
+                       long long strideB, double beta, double* C, long long strideC, int batch, void* stream) {
+  if (!A || !B || !C) return fail(TICC_ERR_VALUE, "dgemm: null pointer argument");
+  return ticc::dgemm_batched(false, false, m, n, k, alpha, A, k, strideA, B, n, strideB, beta, C, n, strideC, batch,
+                             as_stream(stream));
+}
+
+int ticc_profile_update_begin(int sample_every, int max_samples) {
+  if (sample_every < 1 || max_samples < 1 || max_samples > (1 << 20))
+    return fail(TICC_ERR_VALUE, "profile: sample_every and max_samples must be positive");
+  return ticc::profile_begin(sample_every, max_samples);
+}
+
+int ticc_profile_update_end(double* total_ms, int* count) {
+  if (!total_ms || !count) return fail(TICC_ERR_VALUE, "profile: null pointer argument");
+  return ticc::profile_end(total_ms, count);
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+// W(R) contraction and radial-matrix assembly (kernel A).
+//
+// For every radial point r and helicity block K with weighted basis rows
+// G = [B_real | B_imag] (N_K x n_ang each):
+//   V_K(r) = norm (B_real diag(V_r) B_real^T + B_imag diag(V_r) B_imag^T)
+//   W(r)[a][b] = U[a][b] / R_r^2 + 2 mu V[a][b] (+ 2 mu E_int[a] on the diagonal)
+// Cross-K elements of V are zero (W there is U/R^2 only).  One wave computes
+// one 32 x 32 tile (a-tile <= b-tile) with FP64 MFMA, diag(V_r) applied to the
+// A operand as it is loaded, and writes both mirror images, so W is exactly
+// symmetric.
+#include "ticc_common.h"
+
+namespace ticc {
+namespace {
+
+__global__ void k_w_init(const double* __restrict__ U, const double* __restrict__ R, int N, int nR,
+                         double* __restrict__ W) {
+  const size_t n2 = (size_t)N * N;
+  const size_t total = n2 * nR;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(idx / n2);
+    const double RR = R[r];
+    W[idx] = U[idx % n2] / (RR * RR);
+  }
+}
+
+// tiles[t] = (row offset o, block size nk, a-tile, b-tile)
+__global__ __launch_bounds__(64) void k_contract(const int4* __restrict__ tiles, const int* __restrict__ chan,
+                                                 const double* __restrict__ Br, const double* __restrict__ Bi,
+                                                 int n_ang, double norm, const double* __restrict__ R,
+                                                 const double* __restrict__ V, const double* __restrict__ U,
+                                                 const double* __restrict__ Eint, double two_mu, int N,
+                                                 double* __restrict__ W) {
+  const int4 tl = tiles[blockIdx.x];
+  const int r = blockIdx.y;
+  const int o = tl.x, nk = tl.y;
+  const int a0 = 32 * tl.z, b0 = 32 * tl.w;
+  const int lane = threadIdx.x;
+  const int lr = lane & 15;
+  const int lq = lane >> 4;
+  const double* Vr = V + (size_t)r * n_ang;
+
+  double4_t acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+  const double* rows_a[2];
+  const double* rows_b[2];
+  bool va[2], vb[2];
+  for (int part = 0; part < 2; ++part) {
+    const double* Bm = (part == 0) ? Br : Bi;
+    if (Bm == nullptr) break;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int la = a0 + 16 * q + lr;
+      const int lb = b0 + 16 * q + lr;
+      va[q] = la < nk;
+      vb[q] = lb < nk;
+      rows_a[q] = Bm + (size_t)(o + (va[q] ? la : 0)) * n_ang;
+      rows_b[q] = Bm + (size_t)(o + (vb[q] ? lb : 0)) * n_ang;
+    }
+    for (int g0 = 0; g0 < n_ang; g0 += 4) {
+      const int g = g0 + lq;
+      const bool vg = g < n_ang;
+      const double vv = vg ? Vr[g] : 0.0;
+      double a[2], b[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        a[q] = (va[q] && vg) ? rows_a[q][g] * vv : 0.0;
+        b[q] = (vb[q] && vg) ? rows_b[q][g] : 0.0;
+      }
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
+    }
+  }
+
+  const double RR = R[r];
+  const double R2 = RR * RR;
+  double* Wr = W + (size_t)r * N * N;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int la = a0 + 16 * bi + mfma_drow(lq, q);
+        const int lb = b0 + 16 * bj + lr;
+        if (la >= nk || lb >= nk) continue;
+        if (a0 == b0 && la > lb) continue;
+        const int ca = chan[o + la];
+        const int cb = chan[o + lb];
+        double w = U[(size_t)ca * N + cb] / R2 + two_mu * (norm * acc[bi][bj][q]);
+        if (ca == cb) w += two_mu * Eint[ca];
+        Wr[(size_t)ca * N + cb] = w;
+        Wr[(size_t)cb * N + ca] = w;
+      }
+}
+
+}  // namespace
+
+int contract_w(int N, int n_ang, int n_block, const int* block_offsets, const int* block_channels,
+               const double* Br, const double* Bi, double norm, int nR, const double* R, const double* V,
+               const double* U, const double* Eint, double mu, double* W, hipStream_t stream) {
+  // tile list (host) -> workspace (device)
+  int n_tiles = 0;
+  for (int k = 0; k < n_block; ++k) {
+    const int nk = block_offsets[k + 1] - block_offsets[k];
+    const int t = (nk + 31) / 32;
+    n_tiles += t * (t + 1) / 2;
+  }
+  {
+    long long total = (long long)nR * N * N;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_w_init, dim3((int)blocks), dim3(256), 0, stream, U, R, N, nR, W);
+    TICC_CHECK_LAUNCH();
+  }
+  if (n_tiles == 0 || nR == 0) return TICC_OK;
+  int4* host_tiles = new int4[n_tiles];
+  int p = 0;
+  for (int k = 0; k < n_block; ++k) {
+    const int o = block_offsets[k];
+    const int nk = block_offsets[k + 1] - o;
+    const int t = (nk + 31) / 32;
+    for (int ta = 0; ta < t; ++ta)
+      for (int tb = ta; tb < t; ++tb) host_tiles[p++] = make_int4(o, nk, ta, tb);
+  }
+  void* ws = nullptr;
+  hipError_t e0 = hipStreamSynchronize(stream);  // earlier calls may still read the tile list
+  if (e0 != hipSuccess) {
+    delete[] host_tiles;
+    return fail(TICC_ERR_HIP, "stream synchronize failed: %s", hipGetErrorString(e0));
+  }
+  int st = workspace(WS_CONTRACT, sizeof(int4) * n_tiles, &ws);
+  if (st != TICC_OK) {
+    delete[] host_tiles;
+    return st;
+  }
+  hipError_t e = hipMemcpyAsync(ws, host_tiles, sizeof(int4) * n_tiles, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  delete[] host_tiles;
+  if (e != hipSuccess) return fail(TICC_ERR_HIP, "tile upload failed: %s", hipGetErrorString(e));
+  hipLaunchKernelGGL(k_contract, dim3(n_tiles, nR), dim3(64), 0, stream, static_cast<const int4*>(ws), block_channels,
+                     Br, Bi, n_ang, norm, R, V, U, Eint, 2.0 * mu, N, W);
+  TICC_CHECK_LAUNCH();
+  return TICC_OK;
+}
+
+}  // namespace ticc

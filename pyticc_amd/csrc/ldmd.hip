@@ -1,0 +1,315 @@
+// Manolopoulos LDMD sweep (kernel B) for an energy batch.
+//
+// Per sector s with half step h (all matrices FP64, padded to Np):
+//   ref    = diag(W_mid)                          (energy independent)
+//   Q_s    = h (W_start - diag ref) / 3,  Q_e = h (W_end - diag ref) / 3
+//   Q_m    = (4/h) [ (I - h^2 (W_mid - diag ref)/6)^-1 - I ]   (batched over sectors)
+//   per energy e:  p^2 = ref - 2 mu E_e -> y1, y2 (coth/cot, csch/csc or series)
+//     A1 = Y + Q_s + diag y1                 -> X1 = A1^-1   (blocked GJ)
+//     Ym = Q_m + diag y1 - y2 X1 y2
+//     A2 = Ym + Q_m + diag y1                -> X2 = A2^-1
+//     Ye = Q_e + diag y1 - y2 X2 y2,   Y <- (Ye + Ye^T) / 2
+// Q_s and Q_e are never stored: the form kernels read W_start / W_end directly.
+// The elementwise passes fuse the column unpermute of the GJ result.
+#include <cmath>
+
+#include "ticc_common.h"
+
+namespace ticc {
+namespace {
+
+constexpr int QCHUNK = 32;  // sectors whose Q_mid are inverted in one batch
+
+__device__ __forceinline__ void reference_values(double h, double p2, double* y1, double* y2) {
+  const double mag = sqrt(fabs(p2));
+  const double arg = mag * h;
+  if (fabs(arg) < 1.0e-5) {
+    *y1 = 1.0 / h + p2 * h / 3.0 - p2 * p2 * h * h * h / 45.0;
+    *y2 = 1.0 / h - p2 * h / 6.0 + 7.0 * (p2 * p2) * (h * h * h) / 360.0;
+  } else if (p2 >= 0.0) {
+    *y1 = mag / tanh(arg);
+    *y2 = mag / sinh(arg);
+  } else {
+    *y1 = mag / tan(arg);
+    *y2 = mag / sin(arg);
+  }
+}
+
+__global__ void k_refvals(double h, const double* __restrict__ Wm, int N, int Np, const double* __restrict__ E,
+                          double two_mu, int nE, double* __restrict__ y1, double* __restrict__ y2) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nE * Np) return;
+  const int e = idx / Np;
+  const int i = idx % Np;
+  double a = 0.0, b = 0.0;
+  if (i < N) {
+    const double p2 = Wm[(size_t)i * N + i] - two_mu * E[e];
+    reference_values(h, p2, &a, &b);
+  }
+  y1[idx] = a;
+  y2[idx] = b;
+}
+
+// A1 = (Y + Q_s) + diag y1, identity padding.
+__global__ void k_form1(const double* __restrict__ Y, const double* __restrict__ Ws, const double* __restrict__ Wm,
+                        double h, const double* __restrict__ y1, int N, int Np, double* __restrict__ A) {
+  const int e = blockIdx.y;
+  const size_t n2 = (size_t)Np * Np;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n2; idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / Np);
+    const int j = (int)(idx % Np);
+    double v;
+    if (i < N && j < N) {
+      double t = Ws[(size_t)i * N + j];
+      if (i == j) t = t - Wm[(size_t)i * N + i];
+      const double qs = h * t / 3.0;
+      v = Y[e * n2 + idx] + qs;
+      if (i == j) v += y1[(size_t)e * Np + i];
+    } else {
+      v = (i == j) ? 1.0 : 0.0;
+    }
+    A[e * n2 + idx] = v;
+  }
+}
+
+// A2 = (Ym + Q_m) + diag y1,  Ym = (Q_m + diag y1) - y2 (X1 y2).
+__global__ void k_form2(const double* __restrict__ X1, const int* __restrict__ pinv, const double* __restrict__ Qm,
+                        const double* __restrict__ y1, const double* __restrict__ y2, int N, int Np,
+                        double* __restrict__ A) {
+  const int e = blockIdx.y;
+  const size_t n2 = (size_t)Np * Np;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n2; idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / Np);
+    const int j = (int)(idx % Np);
+    double v;
+    if (i < N && j < N) {
+      const double x = X1[e * n2 + (size_t)i * Np + pinv[(size_t)e * Np + j]];
+      const double q = Qm[idx];
+      const double y1i = y1[(size_t)e * Np + i];
+      double ym = q;
+      if (i == j) ym += y1i;
+      ym = ym - y2[(size_t)e * Np + i] * (x * y2[(size_t)e * Np + j]);
+      v = ym + q;
+      if (i == j) v += y1i;
+    } else {
+      v = (i == j) ? 1.0 : 0.0;
+    }
+    A[e * n2 + idx] = v;
+  }
+}
+
+// Y = (Ye + Ye^T)/2,  Ye = (Q_e + diag y1) - y2 (X2 y2).  32 x 32 tiles through LDS.
+__device__ __forceinline__ double yend_elem(const double* __restrict__ X2e, const int* __restrict__ pinve,
+                                            const double* __restrict__ We, const double* __restrict__ Wm, double h,
+                                            const double* __restrict__ y1e, const double* __restrict__ y2e, int N,
+                                            int Np, int i, int j) {
+  double t = We[(size_t)i * N + j];
+  if (i == j) t = t - Wm[(size_t)i * N + i];
+  double ye = h * t / 3.0;
+  if (i == j) ye += y1e[i];
+  const double x = X2e[(size_t)i * Np + pinve[j]];
+  return ye - y2e[i] * (x * y2e[j]);
+}
+
+__global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, const int* __restrict__ pinv,
+                                               const double* __restrict__ We, const double* __restrict__ Wm,
+                                               double h, const double* __restrict__ y1, const double* __restrict__ y2,
+                                               int N, int Np, double* __restrict__ Y) {
+  __shared__ double tile[32][33];
+  const int e = blockIdx.y;
+  const int tn = Np / 32;
+  const int ti = blockIdx.x / tn;
+  const int tj = blockIdx.x % tn;
+  const size_t n2 = (size_t)Np * Np;
+  const double* X2e = X2 + e * n2;
+  const int* pinve = pinv + (size_t)e * Np;
+  const double* y1e = y1 + (size_t)e * Np;
+  const double* y2e = y2 + (size_t)e * Np;
+  const int tx = threadIdx.x & 31;
+  const int ty = threadIdx.x >> 5;  // 0..7
+  // transposed tile: element (tj*32 + r, ti*32 + tx) stored at tile[tx][r]
+  for (int r = ty; r < 32; r += 8) {
+    const int i = tj * 32 + r;
+    const int j = ti * 32 + tx;
+    tile[tx][r] = (i < N && j < N) ? yend_elem(X2e, pinve, We, Wm, h, y1e, y2e, N, Np, i, j) : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = ti * 32 + r;
+    const int j = tj * 32 + tx;
+    double v = 0.0;
+    if (i < N && j < N) {
+      const double a = yend_elem(X2e, pinve, We, Wm, h, y1e, y2e, N, Np, i, j);
+      v = 0.5 * (a + tile[r][tx]);
+    }
+    Y[e * n2 + (size_t)i * Np + j] = v;
+  }
+}
+
+// M_s = I - (h_s^2 T_mid)/6 for a chunk of sectors.
+__global__ void k_qm_form(const double* __restrict__ Wm0, long long stride, const double* __restrict__ hs, int N,
+                          int Np, double* __restrict__ M) {
+  const int s = blockIdx.y;
+  const double h = hs[s];
+  const double* Wm = Wm0 + s * stride;
+  const size_t n2 = (size_t)Np * Np;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n2; idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / Np);
+    const int j = (int)(idx % Np);
+    double v;
+    if (i < N && j < N) {
+      double t = Wm[(size_t)i * N + j];
+      if (i == j) t = t - Wm[(size_t)i * N + i];
+      v = ((i == j) ? 1.0 : 0.0) - (h * h) * t / 6.0;
+    } else {
+      v = (i == j) ? 1.0 : 0.0;
+    }
+    M[s * n2 + idx] = v;
+  }
+}
+
+// Q_m = (4/h) (M^-1 - I).
+__global__ void k_qm_finish(const double* __restrict__ Xi, const int* __restrict__ pinv, const double* __restrict__ hs,
+                            int N, int Np, double* __restrict__ Qm) {
+  const int s = blockIdx.y;
+  const double c = 4.0 / hs[s];
+  const size_t n2 = (size_t)Np * Np;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n2; idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / Np);
+    const int j = (int)(idx % Np);
+    double v = 0.0;
+    if (i < N && j < N) v = c * (Xi[s * n2 + (size_t)i * Np + pinv[(size_t)s * Np + j]] - ((i == j) ? 1.0 : 0.0));
+    Qm[s * n2 + idx] = v;
+  }
+}
+
+__global__ void k_init_inelastic(const double* __restrict__ W0, const double* __restrict__ E, double two_mu, int N,
+                                 int nE, double* __restrict__ Y) {
+  const size_t n2 = (size_t)N * N;
+  const size_t total = n2 * nE;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const int e = (int)(idx / n2);
+    const size_t rem = idx % n2;
+    const int i = (int)(rem / N);
+    const int j = (int)(rem % N);
+    double v = 0.0;
+    if (i == j) v = sqrt(fabs(W0[(size_t)i * N + i] - two_mu * E[e]));
+    Y[idx] = v;
+  }
+}
+
+int grid1(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 2048) b = 2048;
+  return (int)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+int ldmd_init(int nE, int N, const double* W0, const double* E_host, double mu, double* Y, hipStream_t stream) {
+  void* ws = nullptr;
+  TICC_RETURN_IF(workspace(WS_MATCH, sizeof(double) * nE, &ws));
+  double* dE = static_cast<double*>(ws);
+  TICC_CHECK_HIP(hipMemcpyAsync(dE, E_host, sizeof(double) * nE, hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_init_inelastic, dim3(grid1((long long)nE * N * N)), dim3(256), 0, stream, W0, dE, 2.0 * mu, N,
+                     nE, Y);
+  TICC_CHECK_LAUNCH();
+  TICC_CHECK_HIP(hipStreamSynchronize(stream));
+  return TICC_OK;
+}
+
+int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const double* hs_host, const double* Ws0,
+                   const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream) {
+  const int Np = round_up(N, GJ_NB);
+  if (Np > MAX_NP) return fail(TICC_ERR_UNSUPPORTED, "n_channel=%d exceeds the supported maximum %d", N, MAX_NP);
+  const int nb2 = nE > QCHUNK ? nE : QCHUNK;
+  const size_t n2 = (size_t)Np * Np;
+  // workspace layout
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t oY = take(sizeof(double) * nE * n2);
+  const size_t oA0 = take(sizeof(double) * nb2 * n2);
+  const size_t oA1 = take(sizeof(double) * nb2 * n2);
+  const size_t oQ = take(sizeof(double) * QCHUNK * n2);
+  const size_t oP = take(sizeof(double) * nb2 * Np * GJ_NB);
+  const size_t oy1 = take(sizeof(double) * nE * Np);
+  const size_t oy2 = take(sizeof(double) * nE * Np);
+  const size_t oE = take(sizeof(double) * nE);
+  const size_t oH = take(sizeof(double) * nS);
+  const size_t oSrc = take(sizeof(int) * nb2 * Np);
+  const size_t oPiv = take(sizeof(int) * nb2 * GJ_NB);
+  const size_t oPerm = take(sizeof(int) * nb2 * Np);
+  const size_t oPinv = take(sizeof(int) * nb2 * Np);
+  const size_t oFlag = take(sizeof(int));
+  void* base = nullptr;
+  TICC_RETURN_IF(workspace(WS_SWEEP, off, &base));
+  char* w = static_cast<char*>(base);
+  double* Yp = reinterpret_cast<double*>(w + oY);
+  double* A0 = reinterpret_cast<double*>(w + oA0);
+  double* A1 = reinterpret_cast<double*>(w + oA1);
+  double* Qm = reinterpret_cast<double*>(w + oQ);
+  double* Pout = reinterpret_cast<double*>(w + oP);
+  double* y1 = reinterpret_cast<double*>(w + oy1);
+  double* y2 = reinterpret_cast<double*>(w + oy2);
+  double* dE = reinterpret_cast<double*>(w + oE);
+  double* dH = reinterpret_cast<double*>(w + oH);
+  int* src = reinterpret_cast<int*>(w + oSrc);
+  int* piv = reinterpret_cast<int*>(w + oPiv);
+  int* perm = reinterpret_cast<int*>(w + oPerm);
+  int* pinv = reinterpret_cast<int*>(w + oPinv);
+  int* flag = reinterpret_cast<int*>(w + oFlag);
+
+  TICC_CHECK_HIP(hipMemcpyAsync(dE, E_host, sizeof(double) * nE, hipMemcpyHostToDevice, stream));
+  TICC_CHECK_HIP(hipMemcpyAsync(dH, hs_host, sizeof(double) * nS, hipMemcpyHostToDevice, stream));
+  TICC_CHECK_HIP(hipMemsetAsync(flag, 0, sizeof(int), stream));
+  TICC_RETURN_IF(pad_identity(Y, (long long)N * N, N, Yp, Np, nE, stream));
+
+  const double two_mu = 2.0 * mu;
+  const int gelem = grid1((long long)n2);
+  const int tn = Np / 32;
+  for (int s0 = 0; s0 < nS; s0 += QCHUNK) {
+    const int nc = (nS - s0) < QCHUNK ? (nS - s0) : QCHUNK;
+    hipLaunchKernelGGL(k_qm_form, dim3(gelem, nc), dim3(256), 0, stream, Wm0 + s0 * wstride, wstride, dH + s0, N, Np,
+                       A0);
+    TICC_CHECK_LAUNCH();
+    double* res = nullptr;
+    TICC_RETURN_IF(gj_invert_raw(A0, A1, nc, Np, &res, perm, pinv, Pout, src, piv, stream));
+    hipLaunchKernelGGL(k_qm_finish, dim3(gelem, nc), dim3(256), 0, stream, res, pinv, dH + s0, N, Np, Qm);
+    TICC_CHECK_LAUNCH();
+
+    for (int q = 0; q < nc; ++q) {
+      const int s = s0 + q;
+      const double h = hs_host[s];
+      const double* Ws = Ws0 + s * wstride;
+      const double* Wm = Wm0 + s * wstride;
+      const double* We = We0 + s * wstride;
+      hipLaunchKernelGGL(k_refvals, dim3((nE * Np + 255) / 256), dim3(256), 0, stream, h, Wm, N, Np, dE, two_mu, nE,
+                         y1, y2);
+      TICC_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_form1, dim3(gelem, nE), dim3(256), 0, stream, Yp, Ws, Wm, h, y1, N, Np, A0);
+      TICC_CHECK_LAUNCH();
+      double* r1 = nullptr;
+      TICC_RETURN_IF(gj_invert_raw(A0, A1, nE, Np, &r1, perm, pinv, Pout, src, piv, stream));
+      double* other = (r1 == A0) ? A1 : A0;
+      hipLaunchKernelGGL(k_form2, dim3(gelem, nE), dim3(256), 0, stream, r1, pinv, Qm + q * n2, y1, y2, N, Np, other);
+      TICC_CHECK_LAUNCH();
+      double* r2 = nullptr;
+      TICC_RETURN_IF(gj_invert_raw(other, r1, nE, Np, &r2, perm, pinv, Pout, src, piv, stream));
+      hipLaunchKernelGGL(k_form3, dim3(tn * tn, nE), dim3(256), 0, stream, r2, pinv, We, Wm, h, y1, y2, N, Np, Yp);
+      TICC_CHECK_LAUNCH();
+    }
+  }
+  TICC_RETURN_IF(unpad(Yp, Np, Y, (long long)N * N, N, nE, stream));
+  TICC_RETURN_IF(check_finite(Y, (long long)nE * N * N, flag, stream));
+  int hflag = 0;
+  TICC_CHECK_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, stream));
+  TICC_CHECK_HIP(hipStreamSynchronize(stream));
+  if (hflag) return fail(TICC_ERR_NONFINITE, "LDMD sweep produced a non-finite log-derivative matrix");
+  return TICC_OK;
+}
+
+}  // namespace ticc

@@ -1,0 +1,73 @@
+// Shared definitions for the CDNA4 (gfx950) coupled-channel kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/ticc_hip.h"
+
+namespace ticc {
+
+// Record a formatted error message for ticc_last_error(); returns `code`.
+int fail(int code, const char* fmt, ...);
+
+// Device-memory workspace owned by the library, one growable buffer per
+// (device, slot).  Slots let the contraction and the sweep run concurrently
+// from two host threads without sharing scratch memory.
+enum WorkspaceSlot { WS_SWEEP = 0, WS_CONTRACT = 1, WS_MATCH = 2, WS_INVERT = 3, WS_COUNT = 4 };
+int workspace(WorkspaceSlot slot, size_t bytes, void** out);
+
+// Panel width of the blocked Gauss-Jordan inversion and the padding quantum of
+// every internal square matrix (Np = round_up(N, GJ_NB)).
+constexpr int GJ_NB = 32;
+constexpr int MAX_NP = 1024;
+
+inline int round_up(int n, int q) { return (n + q - 1) / q * q; }
+
+}  // namespace ticc
+
+#define TICC_CHECK_HIP(call)                                                                  \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return ticc::fail(TICC_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                  \
+  } while (0)
+
+#define TICC_CHECK_LAUNCH() TICC_CHECK_HIP(hipGetLastError())
+
+#define TICC_RETURN_IF(status)          \
+  do {                                  \
+    int s_ = (status);                  \
+    if (s_ != TICC_OK) return s_;       \
+  } while (0)
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// Output layout of v_mfma_f64_16x16x4f64 on gfx950: lane l holds A[l%16][l/16]
+// and B[l/16][l%16] as inputs; accumulator register r of lane l is
+// D[mfma_drow(l/16, r)][l%16].
+__host__ __device__ constexpr int mfma_drow(int lq, int r) { return lq + 4 * r; }
+
+// Internal entry points shared between translation units.
+namespace ticc {
+// Blocked GJ inversion of `batch` padded Np x Np matrices in place in bufA
+// (bufB is the ping-pong partner).  On return the inverse, up to a pending
+// column permutation, is in *result (== bufA or bufB) and pinv[b*Np + c] gives
+// the column of *result that holds column c of the inverse.
+int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result, int* perm, int* pinv,
+                  double* pout, int* src, int* piv, hipStream_t stream);
+// Gather the column permutation into a plain inverse: out[b][r][c] = in[b][r][pinv[c]].
+int gj_unpermute(const double* in, double* out, int batch, int Np, const int* pinv, hipStream_t stream);
+// Batched C = alpha op(A) op(B) + beta C for padded matrices (m, n, k multiples of 32).
+int dgemm_batched(bool transA, bool transB, int m, int n, int k, double alpha, const double* A, int lda,
+                  long long strideA, const double* B, int ldb, long long strideB, double beta, double* C, int ldc,
+                  long long strideC, int batch, hipStream_t stream);
+// Identity padding: dst (batch, Np, Np) <- src (batch, N, N) with ld N, identity in the padding.
+int pad_identity(const double* src, long long src_stride, int N, double* dst, int Np, int batch, hipStream_t stream);
+int unpad(const double* src, int Np, double* dst, long long dst_stride, int N, int batch, hipStream_t stream);
+// Non-finite scan: *flag (device int) set to 1 if any of n doubles is NaN/Inf.
+int check_finite(const double* x, long long n, int* flag, hipStream_t stream);
+}  // namespace ticc

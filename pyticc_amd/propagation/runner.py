@@ -1,0 +1,72 @@
+"""Window loop of the exact-CC propagation (src/pyticc/propagation/runner.py:112-322).
+
+Each radial window: the device contraction writes W_base at the window's new
+radial points (the shared endpoint of the previous window is reused, as in
+runner.py:258-263), then the device LDMD sweep advances Y, which stays in HBM
+for the whole propagation.
+"""
+
+from __future__ import annotations
+
+from math import ceil
+from time import perf_counter
+
+import numpy as np
+import torch
+
+from pyticc_amd._log import fail, logger
+from pyticc_amd.energy import get_Etot
+from pyticc_amd.propagation.device import resolve_device
+from pyticc_amd.propagation.grid import window_points
+from pyticc_amd.propagation.logd import initial_logD_energies, propagate_points
+
+_MAX_WINDOW = 256
+
+
+def window_size(n_sector: int, n_grid: int, n_channel: int, memory_mb: float) -> int:
+    """Sectors per window: W_base at 2 points per sector plus the PES slice within memory_mb."""
+    per_sector = 2 * (8 * n_channel * n_channel + 8 * n_grid)
+    fixed = 8 * n_channel * n_channel + 8 * n_grid
+    est = max(1, int((memory_mb * 1024**2 - fixed) // per_sector))
+    return max(1, min(n_sector, _MAX_WINDOW, est))
+
+
+def propagate(hamiltonian, Etot, radial_sectors, config) -> torch.Tensor:
+    """Y_BF at the last sector end for every energy, shape (n_E, N, N), on the device."""
+    energies = get_Etot(Etot)
+    if config.mode != "inelastic":
+        raise fail(NotImplementedError, "capture-mode propagation is not supported by the MI355X backend")
+    sectors = tuple(radial_sectors)
+    if not sectors:
+        raise fail(RuntimeError, "Propagation produced no radial windows")
+    resolved = resolve_device(config.device)
+    dev = resolved.torch_device
+    N = hamiltonian.basis.n_channel
+    mu = hamiltonian.reduced_mass
+    size = window_size(len(sectors), hamiltonian.potential_grid_size, N, config.memory_mb)
+    hs_all = np.asarray([s.radial_half_step for s in sectors])
+    progress_every = max(1, ceil(len(sectors) / 10))
+    next_progress = progress_every
+    started = perf_counter()
+    Y = None
+    W_prev = None
+    done = 0
+    for w0 in range(0, len(sectors), size):
+        window = sectors[w0 : w0 + size]
+        pts = window_points(window)
+        Wpts = torch.empty((pts.size, N, N), dtype=torch.float64, device=dev)
+        if W_prev is not None and np.isclose(pts[0], W_prev[0], rtol=0.0, atol=1.0e-12):
+            Wpts[0].copy_(W_prev[1])
+            hamiltonian.W_points(pts[1:], dev, out=Wpts[1:])
+        else:
+            hamiltonian.W_points(pts, dev, out=Wpts)
+        if Y is None:
+            Y = initial_logD_energies(Wpts[0], energies, mu, dev)
+        propagate_points(Y, energies, mu, hs_all[w0 : w0 + len(window)], Wpts, dev)
+        W_prev = (float(pts[-1]), Wpts[-1].clone())
+        done += len(window)
+        if config.print_verbose and (done == len(sectors) or done >= next_progress):
+            logger.info(f"Propagation: {done}/{len(sectors)} sectors, R={pts[-1]:.6f} bohr, wall={perf_counter() - started:.3f} s")
+            while next_progress <= done:
+                next_progress += progress_every
+    return Y

@@ -1,0 +1,44 @@
+"""Shared builders: product setup -> oracle Problem, random coupled W batches."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from oracle.pipeline import Problem
+from pyticc_amd import prepare_potential
+from pyticc_amd.matrix.centrifugal import get_Umat_BF, ladder_labels
+from pyticc_amd.scattering.hamiltonian import build_hamiltonian
+
+
+def oracle_problem(workload):
+    """Prepare the PES grid and the oracle Problem for a synthetic workload."""
+    sysm = workload.system
+    grid = prepare_potential(sysm, workload.boundaries, workload.half_steps, **workload.quadrature)
+    ham = build_hamiltonian(sysm, grid)
+    vb = ham.vbasis
+    blocks = [(np.asarray(vb.channel_indices[K]), vb.B_real[K], None if vb.B_imag is None else vb.B_imag[K]) for K in sorted(vb.channel_indices)]
+    basis = sysm.basis
+    prob = Problem(
+        blocks=blocks,
+        norm=vb.normalization,
+        U=get_Umat_BF(basis),
+        E_int=basis.E_int,
+        mu=sysm.reduced_mass,
+        ladder=ladder_labels(basis),
+        j_couple=np.asarray([c.j_couple for c in basis]),
+        Jtot=basis.Jtot,
+        sectors=[(s.radial_start, s.radial_end) for s in grid.sectors],
+        V=grid.values.reshape(grid.radial_points.size, -1),
+    )
+    return grid, ham, prob
+
+
+def random_W_batch(rng, n_sector, N, scale=1.0, diag_shift=0.0):
+    """Symmetric energy-independent W_base batches (start, mid, end) that vary smoothly."""
+    base = rng.standard_normal((N, N)) * 0.3 * scale
+    base = 0.5 * (base + base.T) + np.diag(rng.uniform(-2.0, 4.0, N) * scale + diag_shift)
+    out = []
+    for _ in range(3):
+        d = rng.standard_normal((n_sector, N, N)) * 0.05 * scale
+        out.append(base[None] + 0.5 * (d + np.swapaxes(d, 1, 2)))
+    return out

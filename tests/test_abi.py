@@ -1,0 +1,37 @@
+"""The C-ABI library loads and exports every symbol declared in include/ticc_hip.h (no GPU needed)."""
+
+import re
+from pathlib import Path
+
+from pyticc_amd import _backend
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "ticc_hip.h"
+
+
+def _declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(ticc_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _backend.load()
+    declared = _declared()
+    assert len(declared) >= 10
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(_backend.EXPORTED_SYMBOLS) == declared
+
+
+def test_abi_version_and_device_count_without_gpu():
+    lib = _backend.load()
+    assert lib.ticc_abi_version() == 1
+    assert _backend.device_count() >= 0
+
+
+def test_invalid_arguments_map_to_value_error():
+    import pytest
+
+    with pytest.raises(ValueError, match="n_channel"):
+        _backend.call("ticc_invert_batched", 1, 0, None, None)
+    with pytest.raises(NotImplementedError):
+        _backend.call("ticc_invert_batched", 1, 5000, None, None)
