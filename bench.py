@@ -10,7 +10,7 @@ LDMD sweep of all 64 energies, and the fused BF->SF + K->S matching.
 value = energies x sectors / seconds per step (energy.R-steps/s).
 roofline: the rank-32 Gauss-Jordan update kernel (the sweep's dominant
 kernel), timed live with HIP events; algorithmic FP64 flop per launch =
-2 N^2 * 32 * n_energy (4N^3 per energy.R-step over 2 N/32 launches).
+2 N^2 * 32 * (matrices in the launch) (4N^3 per energy.R-step over 2 N/32 launches).
 cpu_baseline: the NumPy oracle (a restatement of the reference algorithm) on
 the same system, all 64 energies over the first sectors until >= 10 s of
 work, on this host's cores.
@@ -21,6 +21,7 @@ Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu-baseline]
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -129,12 +130,15 @@ def main() -> int:
     elapsed = time.perf_counter() - t
     tot_ms = np.zeros(1)
     cnt = np.zeros(1, dtype=np.int32)
-    _backend.check(lib.ticc_profile_update_end(tot_ms.ctypes.data_as(_backend._dp), cnt.ctypes.data_as(_backend._ip)))
+    nbat = np.zeros(1, dtype=np.int64)
+    _backend.check(lib.ticc_profile_update_end(tot_ms.ctypes.data_as(_backend._dp), cnt.ctypes.data_as(_backend._ip),
+                                               nbat.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
     ms_per_step = 1e3 * elapsed / args.steps
     value = nE * nS / (elapsed / args.steps)
 
     Np = (N + 31) // 32 * 32
-    flop_per_launch = 2.0 * Np * Np * 32 * nE
+    batch_per_launch = float(nbat[0]) / max(1, int(cnt[0]))
+    flop_per_launch = 2.0 * Np * Np * 32 * batch_per_launch
     avg_ms = float(tot_ms[0]) / max(1, int(cnt[0]))
     achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if cnt[0] else None
     S = unpack_smat(S_re, S_im, n_open)
@@ -177,6 +181,7 @@ def main() -> int:
             "frac": None if achieved is None else achieved / FP64_MFMA_PEAK_TFLOPS,
             "traffic": None,
             "avg_launch_us": avg_ms * 1e3,
+            "matrices_per_launch": batch_per_launch,
             "timed_launches": int(cnt[0]),
         },
         "sweep_tflops_credited": 4.0 * N**3 * nE * nS / (elapsed / args.steps) / 1e12,

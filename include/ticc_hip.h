@@ -161,11 +161,11 @@ int ticc_dgemm_batched(int m, int n, int k, double alpha, const double* A, long 
  * the rank-NB Gauss-Jordan update kernel, the sweep's dominant kernel.  Between
  * begin and end, every sample_every-th batched inversion has each of its
  * update launches bracketed by an event pair (at most max_samples pairs).
- * end synchronises the device and returns the summed kernel time and the
- * number of timed launches.
+ * end synchronises the device and returns the summed kernel time, the
+ * number of timed launches and the summed number of matrices they updated.
  */
 int ticc_profile_update_begin(int sample_every, int max_samples);
-int ticc_profile_update_end(double* total_ms, int* count);
+int ticc_profile_update_end(double* total_ms, int* count, long long* total_batch);
 
 #ifdef __cplusplus
 }

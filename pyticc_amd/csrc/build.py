@@ -31,6 +31,10 @@ FLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    # the Gauss-Jordan panel keeps its rows in registers only if its column
+    # loop is fully unrolled (otherwise the row arrays move to scratch)
+    "-mllvm",
+    "-pragma-unroll-threshold=200000",
     "-Wno-unused-command-line-argument",
     f"-I{REPO / 'include'}",
     f"-I{HERE}",

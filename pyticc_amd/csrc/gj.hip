@@ -5,7 +5,7 @@
 //   1. k_gj_panel   (one workgroup per matrix): in-place Gauss-Jordan on the
 //      Np x NB column panel, pivot rows chosen by partial pivoting among the
 //      rows not yet used (LAPACK idamax order: largest |a|, first position on
-//      ties).  The panel lives in registers, one row per thread.  Outputs the
+//      ties).  The panel lives in registers, up to 2 rows per thread.  Outputs the
 //      transformed panel in output-row order, the row source map of the step
 //      and the NB pivot rows.
 //   2. k_gj_update  (tiles x matrices): the rank-NB update
@@ -52,136 +52,219 @@ __device__ __forceinline__ PivotCand wave_argmax(PivotCand c) {
   return c;
 }
 
-// One workgroup per matrix; thread t owns row t (Np <= THREADS).
-template <int THREADS>
-__global__ __launch_bounds__(THREADS) void k_gj_panel(const double* __restrict__ X, int Np, int k0,
-                                                       double* __restrict__ Pout, int* __restrict__ src,
-                                                       int* __restrict__ piv, int* __restrict__ perm,
-                                                       int* __restrict__ pinv) {
-  constexpr int NWAVE = THREADS / 64;
-  __shared__ int pos2row[THREADS];
-  __shared__ int row2pos[THREADS];
-  __shared__ double prow[NB];
-  __shared__ double red_v[NWAVE];
-  __shared__ int red_p[NWAVE];
-  __shared__ int red_r[NWAVE];
-  __shared__ int s_pr;
+// Wave-wide argmax of (value, position) -- larger value wins, smaller position on
+// ties -- without LDS round trips: rotations within each 16-lane row by DPP
+// (row_ror 8, 4, 2, 1 leave the row maximum in every lane of the row), then the
+// four row results are read with v_readlane and combined in uniform registers.
+template <int CTRL>
+__device__ __forceinline__ void dpp_argmax_step(double& v, int& p) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
+  const int op = __builtin_amdgcn_update_dpp(0, p, CTRL, 0xF, 0xF, false);
+  const double ov = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  if (better(ov, op, v, p)) {
+    v = ov;
+    p = op;
+  }
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ void wave_argmax_dpp(double& v, int& p) {
+  dpp_argmax_step<0x128>(v, p);  // row_ror:8
+  dpp_argmax_step<0x124>(v, p);  // row_ror:4
+  dpp_argmax_step<0x122>(v, p);  // row_ror:2
+  dpp_argmax_step<0x121>(v, p);  // row_ror:1
+  double bv = readlane_f64(v, 0);
+  int bp = __builtin_amdgcn_readlane(p, 0);
+#pragma unroll
+  for (int r = 1; r < 4; ++r) {
+    const double rv = readlane_f64(v, 16 * r);
+    const int rp = __builtin_amdgcn_readlane(p, 16 * r);
+    if (better(rv, rp, bv, bp)) {
+      bv = rv;
+      bp = rp;
+    }
+  }
+  v = bv;
+  p = bp;
+}
+
+// Panel: one workgroup per matrix; thread t owns rows t + THREADS q
+// (q < RPT, Np <= THREADS RPT) with their NB panel values in registers and
+// its rows' current positions.  Per column: each wave reduces its best
+// candidate (largest |a|, first position on ties) and its winning lane
+// publishes the candidate row's values in a double-buffered LDS slot; after
+// the column's single barrier every thread picks the global winner, applies
+// the row swap to its own position records and performs the in-place
+// Gauss-Jordan update, streaming the pivot row from LDS.
+template <int THREADS, int RPT>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 256, THREADS / 256))) void k_gj_panel(const double* __restrict__ X, int Np, int k0,
+                                                      double* __restrict__ Pout, int* __restrict__ src,
+                                                      int* __restrict__ piv, int* __restrict__ perm,
+                                                      int* __restrict__ pinv) {
+  constexpr int NW = THREADS / 64;
+  __shared__ double s_vals[2][NW][NB];
+  __shared__ double s_v[2][NW];
+  __shared__ int s_pos[2][NW];
+  __shared__ int s_row[2][NW];
 
   const int b = blockIdx.x;
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = t >> 6;
   const double* Xb = X + (size_t)b * Np * Np;
-  const bool own = t < Np;
 
-  double v[NB];
-  if (own) {
-    const double2* rp = reinterpret_cast<const double2*>(Xb + (size_t)t * Np + k0);
+  double v[RPT][NB];
+  int pos[RPT];
 #pragma unroll
-    for (int l = 0; l < NB / 2; ++l) {
-      double2 q = rp[l];
-      v[2 * l] = q.x;
-      v[2 * l + 1] = q.y;
+  for (int q = 0; q < RPT; ++q) {
+    const int r = t + THREADS * q;
+    if (r < Np) {
+      const double2* rp = reinterpret_cast<const double2*>(Xb + (size_t)r * Np + k0);
+#pragma unroll
+      for (int l = 0; l < NB / 2; ++l) {
+        const double2 d = rp[l];
+        v[q][2 * l] = d.x;
+        v[q][2 * l + 1] = d.y;
+      }
+      pos[q] = r;
+    } else {
+#pragma unroll
+      for (int l = 0; l < NB; ++l) v[q][l] = 0.0;
+      pos[q] = -1;
     }
-  } else {
-#pragma unroll
-    for (int l = 0; l < NB; ++l) v[l] = 0.0;
   }
-  if (t < THREADS) {
-    pos2row[t] = t;
-    row2pos[t] = t;
-  }
-  __syncthreads();
 
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int c = k0 + j;
-    PivotCand cand{-1.0, 0x7fffffff, -1};
-    if (own) {
-      const int p = row2pos[t];
-      if (p >= c) {
-        cand.v = fabs(v[j]);
-        cand.pos = p;
-        cand.row = t;
+    const int buf = j & 1;
+    // candidate: largest |a| among rows at positions >= c (NaN ranks as +inf so
+    // a column always has a pivot and the NaN propagates to the result)
+    double cv = -1.0;
+    int cp = 0x7fffffff;
+    int cq = -1;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      if (pos[q] >= c) {
+        double a = fabs(v[q][j]);
+        if (a != a) a = __builtin_inf();
+        if (better(a, pos[q], cv, cp)) {
+          cv = a;
+          cp = pos[q];
+          cq = q;
+        }
       }
     }
-    cand = wave_argmax(cand);
+    double wv = cv;
+    int wp = cp;
+    wave_argmax_dpp(wv, wp);
+    if (cq >= 0 && cp == wp) {  // this lane holds the wave's winning row
+#pragma unroll
+      for (int q = 0; q < RPT; ++q)
+        if (q == cq) {
+#pragma unroll
+          for (int l = 0; l < NB; ++l) s_vals[buf][wave][l] = v[q][l];
+          s_row[buf][wave] = t + THREADS * q;
+        }
+    }
     if (lane == 0) {
-      red_v[wave] = cand.v;
-      red_p[wave] = cand.pos;
-      red_r[wave] = cand.row;
+      s_v[buf][wave] = wv;
+      s_pos[buf][wave] = wp;
     }
     __syncthreads();
-    if (t == 0) {
-      PivotCand best{red_v[0], red_p[0], red_r[0]};
-      for (int w = 1; w < NWAVE; ++w)
-        if (better(red_v[w], red_p[w], best.v, best.pos)) best = PivotCand{red_v[w], red_p[w], red_r[w]};
-      if (best.row < 0) {  // no finite candidate (NaN column): keep the current row
-        best.row = pos2row[c];
-        best.pos = c;
+    double best_v = s_v[buf][0];
+    int bp = s_pos[buf][0];
+    int bw = 0;
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+      if (better(s_v[buf][w], s_pos[buf][w], best_v, bp)) {
+        best_v = s_v[buf][w];
+        bp = s_pos[buf][w];
+        bw = w;
       }
-      const int pr = best.row;
-      const int bp = best.pos;
-      const int rc = pos2row[c];
-      pos2row[c] = pr;
-      pos2row[bp] = rc;
-      row2pos[pr] = c;
-      row2pos[rc] = bp;
-      s_pr = pr;
     }
-    __syncthreads();
-    const int pr = s_pr;
-    if (t == pr) {
+    const int pr = s_row[buf][bw];
+    const double* prow = s_vals[buf][bw];
+    const double inv = 1.0 / prow[j];
+    // non-pivot rows: a_rl -= (a_rj / p) p_l, a_rj = -a_rj / p; the pivot row is
+    // rescaled afterwards in a branch taken by a single lane.
+    double g[RPT];
 #pragma unroll
-      for (int l = 0; l < NB; ++l) prow[l] = v[l];
+    for (int q = 0; q < RPT; ++q) {
+      if (pos[q] == c) pos[q] = bp;
+      g[q] = v[q][j] * inv;
     }
-    __syncthreads();
-    if (own) {
-      const double inv = 1.0 / prow[j];
-      if (t == pr) {
 #pragma unroll
-        for (int l = 0; l < NB; ++l) v[l] = (l == j) ? inv : prow[l] * inv;
-      } else {
-        const double f = v[j];
+    for (int l = 0; l < NB; ++l) {
+      const double pl = prow[l];
 #pragma unroll
-        for (int l = 0; l < NB; ++l) v[l] = (l == j) ? -f * inv : v[l] - f * (prow[l] * inv);
+      for (int q = 0; q < RPT; ++q) v[q][l] = fma(-g[q], pl, v[q][l]);
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      v[q][j] = -g[q];
+      if (t + THREADS * q == pr) {
+        pos[q] = c;
+#pragma unroll
+        for (int l = 0; l < NB; ++l) v[q][l] = prow[l] * inv;
+        v[q][j] = inv;
       }
     }
   }
-  __syncthreads();
 
   // Outputs in output-position order.
-  if (own) {
-    const int p = row2pos[t];
+  int pold[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = t + THREADS * q;
+    pold[q] = 0;
+    if (pos[q] < 0) continue;
+    const int p = pos[q];
     double2* op = reinterpret_cast<double2*>(Pout + ((size_t)b * Np + p) * NB);
 #pragma unroll
-    for (int l = 0; l < NB / 2; ++l) op[l] = make_double2(v[2 * l], v[2 * l + 1]);
-    src[(size_t)b * Np + t] = pos2row[t];
+    for (int l = 0; l < NB / 2; ++l) op[l] = make_double2(v[q][2 * l], v[q][2 * l + 1]);
+    src[(size_t)b * Np + p] = r;
+    if (p >= k0 && p < k0 + NB) piv[b * NB + (p - k0)] = r;
+    pold[q] = (k0 == 0) ? r : perm[(size_t)b * Np + r];
   }
-  if (t < NB) piv[b * NB + t] = pos2row[k0 + t];
-  int pnew = 0;
-  if (own) pnew = (k0 == 0) ? pos2row[t] : perm[(size_t)b * Np + pos2row[t]];
   __syncthreads();
-  if (own) {
-    perm[(size_t)b * Np + t] = pnew;
-    if (k0 + NB >= Np) pinv[(size_t)b * Np + pnew] = t;
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    if (pos[q] < 0) continue;
+    perm[(size_t)b * Np + pos[q]] = pold[q];
+    if (k0 + NB >= Np) pinv[(size_t)b * Np + pold[q]] = pos[q];
   }
 }
 
-// Update: 256 threads = 4 waves, each wave one 32 x 32 output tile of a 64 x 64
-// workgroup tile; FP64 MFMA 16x16x4 (2 x 2 blocks per wave, K = NB).
-// MFMA operand layout: see mfma_drow in ticc_common.h.
+// Update: 256 threads = 4 waves on a 64 x 64 output tile, each wave 32 x 32
+// (2 x 2 FP64 MFMA 16x16x4 blocks, K = NB).  The Pout rows and the NB pivot
+// rows of the tile are staged through LDS with coalesced loads; the gathered
+// input rows X[src(i)] are loaded straight into the accumulators before the
+// barrier.  Panel columns are overwritten with the transformed panel.
 __global__ __launch_bounds__(256) void k_gj_update(const double* __restrict__ X, double* __restrict__ Out, int Np,
                                                    int k0, const double* __restrict__ Pout,
                                                    const int* __restrict__ src, const int* __restrict__ piv) {
+  __shared__ double sA[64][NB + 1];
+  __shared__ double sB[NB][64];
   const int tiles = (Np + 63) / 64;
-  const int tile = blockIdx.x;
+  const int I0 = (blockIdx.x / tiles) * 64;
+  const int J0 = (blockIdx.x % tiles) * 64;
   const int b = blockIdx.y;
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int i0 = (tile / tiles) * 64 + (wave >> 1) * 32;
-  const int j0 = (tile % tiles) * 64 + (wave & 1) * 32;
-  if (i0 >= Np || j0 >= Np) return;
+  const int t = threadIdx.x;
+  const int wave = t >> 6;
+  const int lane = t & 63;
+  const int i0 = I0 + (wave >> 1) * 32;
+  const int j0 = J0 + (wave & 1) * 32;
+  const bool active = (i0 < Np) && (j0 < Np);
+  const bool panel = (j0 == k0);
 
   const double* Xb = X + (size_t)b * Np * Np;
   double* Ob = Out + (size_t)b * Np * Np;
@@ -191,60 +274,84 @@ __global__ __launch_bounds__(256) void k_gj_update(const double* __restrict__ X,
   const int lr = lane & 15;
   const int lq = lane >> 4;
 
-  if (j0 == k0) {
-    // panel columns: copy the transformed panel.
+  // stage A = Pout[I0:I0+64][0:NB] and B = X[piv][J0:J0+64]
+  {
+    const int row = t >> 2, part = t & 3;
+    const int gi = I0 + row;
+    double2 a[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) a[h] = (gi < Np) ? reinterpret_cast<const double2*>(Pb + (size_t)gi * NB + part * 8)[h] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      sA[row][part * 8 + 2 * h] = a[h].x;
+      sA[row][part * 8 + 2 * h + 1] = a[h].y;
+    }
+    const int k = t >> 3, seg = t & 7;
+    const int gj = J0 + seg * 8;
+    const double* brow = Xb + (size_t)pivb[k] * Np + gj;
+    double2 bb[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) bb[h] = (gj < Np) ? reinterpret_cast<const double2*>(brow)[h] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      sB[k][seg * 8 + 2 * h] = bb[h].x;
+      sB[k][seg * 8 + 2 * h + 1] = bb[h].y;
+    }
+  }
+
+  double4_t acc[2][2];
+  if (active && !panel) {
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = i0 + 16 * bi + mfma_drow(lq, r);
+        const bool pivblk = (row >= k0) && (row < k0 + NB);
+        const double* xr = Xb + (size_t)srcb[row] * Np + j0 + lr;
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) acc[bi][bj][r] = pivblk ? 0.0 : xr[16 * bj];
+      }
+  }
+  __syncthreads();
+  if (!active) return;
+  const int li0 = i0 - I0;
+  const int lj0 = j0 - J0;
+  if (panel) {
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) {
-          const int row = i0 + 16 * bi + mfma_drow(lq, r);
+          const int lrow = li0 + 16 * bi + mfma_drow(lq, r);
           const int col = 16 * bj + lr;
-          Ob[(size_t)row * Np + k0 + col] = Pb[(size_t)row * NB + col];
+          Ob[(size_t)(I0 + lrow) * Np + k0 + col] = sA[lrow][col];
         }
     return;
   }
-
-  double4_t acc[2][2];
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = i0 + 16 * bi + mfma_drow(lq, r);
-        const int col = j0 + 16 * bj + lr;
-        const bool pivblk = (row >= k0) && (row < k0 + NB);
-        acc[bi][bj][r] = pivblk ? 0.0 : Xb[(size_t)srcb[row] * Np + col];
-      }
-
 #pragma unroll
   for (int kk = 0; kk < NB / 4; ++kk) {
-    const int kcol = 4 * kk + lq;
-    const int prow_idx = pivb[kcol];
+    const int kq = 4 * kk + lq;
     double a[2], bv[2];
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi) a[bi] = Pb[(size_t)(i0 + 16 * bi + lr) * NB + kcol];
+    for (int bi = 0; bi < 2; ++bi) a[bi] = sA[li0 + 16 * bi + lr][kq];
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) bv[bj] = Xb[(size_t)prow_idx * Np + j0 + 16 * bj + lr];
+    for (int bj = 0; bj < 2; ++bj) bv[bj] = sB[kq][lj0 + 16 * bj + lr];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
       for (int bj = 0; bj < 2; ++bj)
         acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], bv[bj], acc[bi][bj], 0, 0, 0);
   }
-
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
+    for (int r = 0; r < 4; ++r) {
+      const int row = i0 + 16 * bi + mfma_drow(lq, r);
+      double* orow = Ob + (size_t)row * Np + j0 + lr;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = i0 + 16 * bi + mfma_drow(lq, r);
-        const int col = j0 + 16 * bj + lr;
-        Ob[(size_t)row * Np + col] = acc[bi][bj][r];
-      }
+      for (int bj = 0; bj < 2; ++bj) orow[16 * bj] = acc[bi][bj][r];
+    }
 }
 
 __global__ void k_unpermute(const double* __restrict__ in, double* __restrict__ out, int Np,
@@ -266,6 +373,7 @@ struct UpdateProfile {
   int every = 1;
   long long calls = 0;
   std::vector<hipEvent_t> events;  // pairs: [2i] start, [2i+1] stop
+  std::vector<int> batch;          // matrices in each timed launch
   size_t used = 0;
 };
 UpdateProfile g_prof;
@@ -282,22 +390,26 @@ int profile_begin(int every, int max_pairs) {
     TICC_CHECK_HIP(hipEventCreate(&e));
     g_prof.events.push_back(e);
   }
+  g_prof.batch.resize(g_prof.events.size() / 2);
   g_prof.on = true;
   return TICC_OK;
 }
 
-int profile_end(double* total_ms, int* count) {
+int profile_end(double* total_ms, int* count, long long* total_batch) {
   std::lock_guard<std::mutex> lock(g_prof.mu);
   g_prof.on = false;
   TICC_CHECK_HIP(hipDeviceSynchronize());
   double tot = 0.0;
+  long long nb = 0;
   for (size_t i = 0; i < g_prof.used; ++i) {
+    nb += g_prof.batch[i];
     float ms = 0.0f;
     TICC_CHECK_HIP(hipEventElapsedTime(&ms, g_prof.events[2 * i], g_prof.events[2 * i + 1]));
     tot += ms;
   }
   *total_ms = tot;
   *count = (int)g_prof.used;
+  *total_batch = nb;
   g_prof.used = 0;
   return TICC_OK;
 }
@@ -317,14 +429,19 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
   }
   for (int k0 = 0; k0 < Np; k0 += NB) {
     if (Np <= 256)
-      hipLaunchKernelGGL(k_gj_panel<256>, dim3(batch), dim3(256), 0, stream, cur, Np, k0, pout, src, piv, perm, pinv);
+      hipLaunchKernelGGL((k_gj_panel<256, 1>), dim3(batch), dim3(256), 0, stream, cur, Np, k0, pout, src, piv, perm,
+                         pinv);
+    else if (Np <= 512)
+      hipLaunchKernelGGL((k_gj_panel<512, 1>), dim3(batch), dim3(512), 0, stream, cur, Np, k0, pout, src, piv, perm,
+                         pinv);
     else
-      hipLaunchKernelGGL(k_gj_panel<1024>, dim3(batch), dim3(1024), 0, stream, cur, Np, k0, pout, src, piv, perm,
+      hipLaunchKernelGGL((k_gj_panel<512, 2>), dim3(batch), dim3(512), 0, stream, cur, Np, k0, pout, src, piv, perm,
                          pinv);
     TICC_CHECK_LAUNCH();
     size_t slot = 0;
     if (sample) {
       slot = g_prof.used++;
+      g_prof.batch[slot] = batch;
       TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
     }
     hipLaunchKernelGGL(k_gj_update, dim3(tiles * tiles, batch), dim3(256), 0, stream, cur, nxt, Np, k0, pout, src,

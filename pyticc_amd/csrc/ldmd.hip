@@ -12,13 +12,15 @@
 // Q_s and Q_e are never stored: the form kernels read W_start / W_end directly.
 // The elementwise passes fuse the column unpermute of the GJ result.
 #include <cmath>
+#include <cstdlib>
 
 #include "ticc_common.h"
 
 namespace ticc {
 namespace {
 
-constexpr int QCHUNK = 32;  // sectors whose Q_mid are inverted in one batch
+constexpr int QCHUNK = 32;     // sectors whose Q_mid are inverted in one batch
+constexpr int MAX_GROUPS = 4;  // energy groups swept on separate streams
 
 __device__ __forceinline__ void reference_values(double h, double p2, double* y1, double* y2) {
   const double mag = sqrt(fabs(p2));
@@ -222,9 +224,16 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
                    const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream) {
   const int Np = round_up(N, GJ_NB);
   if (Np > MAX_NP) return fail(TICC_ERR_UNSUPPORTED, "n_channel=%d exceeds the supported maximum %d", N, MAX_NP);
-  const int nb2 = nE > QCHUNK ? nE : QCHUNK;
+  // Energy groups may run on their own streams (TICC_SWEEP_GROUPS); one group
+  // is the default: the panel workgroup needs a whole CU's registers, so it
+  // does not co-run with another group's update.
+  int G = 1;
+  if (const char* env = getenv("TICC_SWEEP_GROUPS")) {  // tuning knob: 1..MAX_GROUPS
+    const int v = atoi(env);
+    if (v >= 1 && v <= MAX_GROUPS) G = v < nE ? v : nE;
+  }
+  const int per = (nE + G - 1) / G;
   const size_t n2 = (size_t)Np * Np;
-  // workspace layout
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
@@ -232,36 +241,49 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
     return o;
   };
   const size_t oY = take(sizeof(double) * nE * n2);
-  const size_t oA0 = take(sizeof(double) * nb2 * n2);
-  const size_t oA1 = take(sizeof(double) * nb2 * n2);
-  const size_t oQ = take(sizeof(double) * QCHUNK * n2);
-  const size_t oP = take(sizeof(double) * nb2 * Np * GJ_NB);
-  const size_t oy1 = take(sizeof(double) * nE * Np);
-  const size_t oy2 = take(sizeof(double) * nE * Np);
+  const size_t oQ = take(sizeof(double) * 2 * QCHUNK * n2);
+  const size_t oM = take(sizeof(double) * 2 * QCHUNK * n2);
   const size_t oE = take(sizeof(double) * nE);
   const size_t oH = take(sizeof(double) * nS);
-  const size_t oSrc = take(sizeof(int) * nb2 * Np);
-  const size_t oPiv = take(sizeof(int) * nb2 * GJ_NB);
-  const size_t oPerm = take(sizeof(int) * nb2 * Np);
-  const size_t oPinv = take(sizeof(int) * nb2 * Np);
   const size_t oFlag = take(sizeof(int));
+  struct Scratch {
+    size_t A0, A1, P, src, piv, perm, pinv, y1, y2;
+  };
+  auto scratch = [&](int batch, bool yv) {
+    Scratch sc{};
+    sc.A0 = yv ? take(sizeof(double) * batch * n2) : 0;
+    sc.A1 = yv ? take(sizeof(double) * batch * n2) : 0;
+    sc.P = take(sizeof(double) * batch * Np * GJ_NB);
+    sc.src = take(sizeof(int) * batch * Np);
+    sc.piv = take(sizeof(int) * batch * GJ_NB);
+    sc.perm = take(sizeof(int) * batch * Np);
+    sc.pinv = take(sizeof(int) * batch * Np);
+    sc.y1 = yv ? take(sizeof(double) * batch * Np) : 0;
+    sc.y2 = yv ? take(sizeof(double) * batch * Np) : 0;
+    return sc;
+  };
+  const Scratch sm = scratch(QCHUNK, false);
+  Scratch sg[MAX_GROUPS];
+  for (int g = 0; g < G; ++g) sg[g] = scratch(per, true);
   void* base = nullptr;
   TICC_RETURN_IF(workspace(WS_SWEEP, off, &base));
   char* w = static_cast<char*>(base);
-  double* Yp = reinterpret_cast<double*>(w + oY);
-  double* A0 = reinterpret_cast<double*>(w + oA0);
-  double* A1 = reinterpret_cast<double*>(w + oA1);
-  double* Qm = reinterpret_cast<double*>(w + oQ);
-  double* Pout = reinterpret_cast<double*>(w + oP);
-  double* y1 = reinterpret_cast<double*>(w + oy1);
-  double* y2 = reinterpret_cast<double*>(w + oy2);
-  double* dE = reinterpret_cast<double*>(w + oE);
-  double* dH = reinterpret_cast<double*>(w + oH);
-  int* src = reinterpret_cast<int*>(w + oSrc);
-  int* piv = reinterpret_cast<int*>(w + oPiv);
-  int* perm = reinterpret_cast<int*>(w + oPerm);
-  int* pinv = reinterpret_cast<int*>(w + oPinv);
-  int* flag = reinterpret_cast<int*>(w + oFlag);
+  auto D = [&](size_t o) { return reinterpret_cast<double*>(w + o); };
+  auto I = [&](size_t o) { return reinterpret_cast<int*>(w + o); };
+  double* Yp = D(oY);
+  double* Qm = D(oQ);
+  double* M0 = D(oM);
+  double* M1 = M0 + QCHUNK * n2;
+  double* dE = D(oE);
+  double* dH = D(oH);
+  int* flag = I(oFlag);
+
+  hipStream_t gs[MAX_GROUPS];
+  hipEvent_t ev[2 + 2 * MAX_GROUPS];
+  TICC_RETURN_IF(aux_streams(G, gs));
+  TICC_RETURN_IF(aux_events(2 + 2 * G, ev));
+  hipEvent_t* ev_qm = ev;             // [2] Q_mid chunk ready (per buffer)
+  hipEvent_t* ev_done = ev + 2;       // [G][2] group finished a chunk (per buffer)
 
   TICC_CHECK_HIP(hipMemcpyAsync(dE, E_host, sizeof(double) * nE, hipMemcpyHostToDevice, stream));
   TICC_CHECK_HIP(hipMemcpyAsync(dH, hs_host, sizeof(double) * nS, hipMemcpyHostToDevice, stream));
@@ -271,38 +293,64 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   const double two_mu = 2.0 * mu;
   const int gelem = grid1((long long)n2);
   const int tn = Np / 32;
-  for (int s0 = 0; s0 < nS; s0 += QCHUNK) {
+  const int n_chunk = (nS + QCHUNK - 1) / QCHUNK;
+  for (int ci = 0; ci < n_chunk; ++ci) {
+    const int s0 = ci * QCHUNK;
     const int nc = (nS - s0) < QCHUNK ? (nS - s0) : QCHUNK;
+    const int qb = ci & 1;
+    double* Qbuf = Qm + (size_t)qb * QCHUNK * n2;
+    if (ci >= 2)  // the groups are done with the chunk that last used this buffer
+      for (int g = 0; g < G; ++g) TICC_CHECK_HIP(hipStreamWaitEvent(stream, ev_done[2 * g + qb], 0));
     hipLaunchKernelGGL(k_qm_form, dim3(gelem, nc), dim3(256), 0, stream, Wm0 + s0 * wstride, wstride, dH + s0, N, Np,
-                       A0);
+                       M0);
     TICC_CHECK_LAUNCH();
     double* res = nullptr;
-    TICC_RETURN_IF(gj_invert_raw(A0, A1, nc, Np, &res, perm, pinv, Pout, src, piv, stream));
-    hipLaunchKernelGGL(k_qm_finish, dim3(gelem, nc), dim3(256), 0, stream, res, pinv, dH + s0, N, Np, Qm);
+    TICC_RETURN_IF(gj_invert_raw(M0, M1, nc, Np, &res, I(sm.perm), I(sm.pinv), D(sm.P), I(sm.src), I(sm.piv), stream));
+    hipLaunchKernelGGL(k_qm_finish, dim3(gelem, nc), dim3(256), 0, stream, res, I(sm.pinv), dH + s0, N, Np, Qbuf);
     TICC_CHECK_LAUNCH();
+    TICC_CHECK_HIP(hipEventRecord(ev_qm[qb], stream));
 
-    for (int q = 0; q < nc; ++q) {
-      const int s = s0 + q;
-      const double h = hs_host[s];
-      const double* Ws = Ws0 + s * wstride;
-      const double* Wm = Wm0 + s * wstride;
-      const double* We = We0 + s * wstride;
-      hipLaunchKernelGGL(k_refvals, dim3((nE * Np + 255) / 256), dim3(256), 0, stream, h, Wm, N, Np, dE, two_mu, nE,
-                         y1, y2);
-      TICC_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_form1, dim3(gelem, nE), dim3(256), 0, stream, Yp, Ws, Wm, h, y1, N, Np, A0);
-      TICC_CHECK_LAUNCH();
-      double* r1 = nullptr;
-      TICC_RETURN_IF(gj_invert_raw(A0, A1, nE, Np, &r1, perm, pinv, Pout, src, piv, stream));
-      double* other = (r1 == A0) ? A1 : A0;
-      hipLaunchKernelGGL(k_form2, dim3(gelem, nE), dim3(256), 0, stream, r1, pinv, Qm + q * n2, y1, y2, N, Np, other);
-      TICC_CHECK_LAUNCH();
-      double* r2 = nullptr;
-      TICC_RETURN_IF(gj_invert_raw(other, r1, nE, Np, &r2, perm, pinv, Pout, src, piv, stream));
-      hipLaunchKernelGGL(k_form3, dim3(tn * tn, nE), dim3(256), 0, stream, r2, pinv, We, Wm, h, y1, y2, N, Np, Yp);
-      TICC_CHECK_LAUNCH();
+    for (int g = 0; g < G; ++g) {
+      const int e0 = g * per;
+      const int ne = (nE - e0) < per ? (nE - e0) : per;
+      if (ne <= 0) continue;
+      hipStream_t st = gs[g];
+      const Scratch& sc = sg[g];
+      double* A0 = D(sc.A0);
+      double* A1 = D(sc.A1);
+      double* y1 = D(sc.y1);
+      double* y2 = D(sc.y2);
+      double* Yg = Yp + (size_t)e0 * n2;
+      TICC_CHECK_HIP(hipStreamWaitEvent(st, ev_qm[qb], 0));
+      for (int q = 0; q < nc; ++q) {
+        const int s = s0 + q;
+        const double h = hs_host[s];
+        const double* Ws = Ws0 + s * wstride;
+        const double* Wm = Wm0 + s * wstride;
+        const double* We = We0 + s * wstride;
+        hipLaunchKernelGGL(k_refvals, dim3((ne * Np + 255) / 256), dim3(256), 0, st, h, Wm, N, Np, dE + e0, two_mu, ne,
+                           y1, y2);
+        TICC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_form1, dim3(gelem, ne), dim3(256), 0, st, Yg, Ws, Wm, h, y1, N, Np, A0);
+        TICC_CHECK_LAUNCH();
+        double* r1 = nullptr;
+        TICC_RETURN_IF(gj_invert_raw(A0, A1, ne, Np, &r1, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src), I(sc.piv), st));
+        double* other = (r1 == A0) ? A1 : A0;
+        hipLaunchKernelGGL(k_form2, dim3(gelem, ne), dim3(256), 0, st, r1, I(sc.pinv), Qbuf + q * n2, y1, y2, N, Np,
+                           other);
+        TICC_CHECK_LAUNCH();
+        double* r2 = nullptr;
+        TICC_RETURN_IF(gj_invert_raw(other, r1, ne, Np, &r2, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src), I(sc.piv), st));
+        hipLaunchKernelGGL(k_form3, dim3(tn * tn, ne), dim3(256), 0, st, r2, I(sc.pinv), We, Wm, h, y1, y2, N, Np, Yg);
+        TICC_CHECK_LAUNCH();
+      }
+      TICC_CHECK_HIP(hipEventRecord(ev_done[2 * g + qb], st));
     }
   }
+  // join: the caller's stream waits for every group's last chunk
+  const int last = (n_chunk - 1) & 1;
+  for (int g = 0; g < G; ++g)
+    if (g * per < nE) TICC_CHECK_HIP(hipStreamWaitEvent(stream, ev_done[2 * g + last], 0));
   TICC_RETURN_IF(unpad(Yp, Np, Y, (long long)N * N, N, nE, stream));
   TICC_RETURN_IF(check_finite(Y, (long long)nE * N * N, flag, stream));
   int hflag = 0;

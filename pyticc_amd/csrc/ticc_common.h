@@ -18,6 +18,10 @@ int fail(int code, const char* fmt, ...);
 // from two host threads without sharing scratch memory.
 enum WorkspaceSlot { WS_SWEEP = 0, WS_CONTRACT = 1, WS_MATCH = 2, WS_INVERT = 3, WS_COUNT = 4 };
 int workspace(WorkspaceSlot slot, size_t bytes, void** out);
+// Library-owned non-blocking streams (default priority) and events of the
+// current device, created once and reused; callers must not destroy them.
+int aux_streams(int n, hipStream_t* out);
+int aux_events(int n, hipEvent_t* out);
 
 // Panel width of the blocked Gauss-Jordan inversion and the padding quantum of
 // every internal square matrix (Np = round_up(N, GJ_NB)).

@@ -18,6 +18,9 @@ struct Buffer {
 constexpr int kMaxDevices = 64;
 Buffer g_ws[kMaxDevices][WS_COUNT];
 std::mutex g_ws_mutex;
+constexpr int kMaxAux = 16;
+hipStream_t g_streams[kMaxDevices][kMaxAux] = {};
+hipEvent_t g_events[kMaxDevices][kMaxAux * 4] = {};
 
 __global__ void k_pad_identity(const double* __restrict__ src, long long src_stride, int N, double* __restrict__ dst,
                                int Np) {
@@ -93,6 +96,30 @@ int workspace(WorkspaceSlot slot, size_t bytes, void** out) {
     buf.size = want;
   }
   *out = buf.ptr;
+  return TICC_OK;
+}
+
+int aux_streams(int n, hipStream_t* out) {
+  int dev = 0;
+  TICC_CHECK_HIP(hipGetDevice(&dev));
+  if (n > kMaxAux || dev >= kMaxDevices) return fail(TICC_ERR_VALUE, "too many auxiliary streams requested");
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  for (int i = 0; i < n; ++i) {
+    if (!g_streams[dev][i]) TICC_CHECK_HIP(hipStreamCreateWithFlags(&g_streams[dev][i], hipStreamNonBlocking));
+    out[i] = g_streams[dev][i];
+  }
+  return TICC_OK;
+}
+
+int aux_events(int n, hipEvent_t* out) {
+  int dev = 0;
+  TICC_CHECK_HIP(hipGetDevice(&dev));
+  if (n > kMaxAux * 4 || dev >= kMaxDevices) return fail(TICC_ERR_VALUE, "too many auxiliary events requested");
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  for (int i = 0; i < n; ++i) {
+    if (!g_events[dev][i]) TICC_CHECK_HIP(hipEventCreateWithFlags(&g_events[dev][i], hipEventDisableTiming));
+    out[i] = g_events[dev][i];
+  }
   return TICC_OK;
 }
 
