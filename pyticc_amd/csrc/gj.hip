@@ -105,7 +105,7 @@ __device__ __forceinline__ void wave_argmax_dpp(double& v, int& p) {
 // the row swap to its own position records and performs the in-place
 // Gauss-Jordan update, streaming the pivot row from LDS.
 template <int THREADS, int RPT>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 256, THREADS / 256))) void k_gj_panel(const double* __restrict__ X, int Np, int k0,
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 256, THREADS / 256))) void k_gj_panel(const double* __restrict__ X, long long bstride, int ld, int col0, int Np, int k0,
                                                       double* __restrict__ Pout, int* __restrict__ src,
                                                       int* __restrict__ piv, int* __restrict__ perm,
                                                       int* __restrict__ pinv) {
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = t >> 6;
-  const double* Xb = X + (size_t)b * Np * Np;
+  const double* Xb = X + (size_t)b * bstride;
 
   double v[RPT][NB];
   int pos[RPT];
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
   for (int q = 0; q < RPT; ++q) {
     const int r = t + THREADS * q;
     if (r < Np) {
-      const double2* rp = reinterpret_cast<const double2*>(Xb + (size_t)r * Np + k0);
+      const double2* rp = reinterpret_cast<const double2*>(Xb + (size_t)r * ld + col0);
 #pragma unroll
       for (int l = 0; l < NB / 2; ++l) {
         const double2 d = rp[l];
@@ -354,6 +354,160 @@ __global__ __launch_bounds__(256) void k_gj_update(const double* __restrict__ X,
     }
 }
 
+// Small panel GEMM (FP64 MFMA) used by the paired block step:
+//   Out[i][0:NB] = mask(i) C[gC(i)][colC : colC+NB] + A[i][0:NB] . B[gB(r)][colB : colB+NB],
+//   mask(i) = 0 for i in [mlo, mlo+NB), 1 otherwise.
+// One workgroup (4 waves) per 64 rows of one matrix, each wave 16 rows x NB.
+__global__ __launch_bounds__(256) void k_panel_gemm(const double* __restrict__ A, const double* __restrict__ C,
+                                                    long long cb, int ldC, int colC, const int* __restrict__ gC,
+                                                    const double* __restrict__ Bm, long long bb, int ldB, int colB,
+                                                    const int* __restrict__ gB, int mlo, int Np,
+                                                    double* __restrict__ Out) {
+  __shared__ double sB[NB][NB + 1];
+  const int b = blockIdx.y;
+  const int t = threadIdx.x;
+  const int wave = t >> 6, lane = t & 63, lr = lane & 15, lq = lane >> 4;
+  const double* Ab = A + (size_t)b * Np * NB;
+  const double* Cb = C + (size_t)b * cb;
+  const double* Bb = Bm + (size_t)b * bb;
+  const int* gCb = gC + (size_t)b * Np;
+  const int* gBb = gB + b * NB;
+  {  // stage B: NB gathered rows x NB columns
+    const int r = t >> 3, seg = t & 7;  // 32 rows x 8 segments of 4
+    const double* br = Bb + (size_t)gBb[r] * ldB + colB + seg * 4;
+#pragma unroll
+    for (int h = 0; h < 4; ++h) sB[r][seg * 4 + h] = br[h];
+  }
+  const int row0 = blockIdx.x * 64 + wave * 16;
+  double4_t acc[2];
+  if (row0 < Np) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = row0 + mfma_drow(lq, r);
+      const bool m = (row >= mlo) && (row < mlo + NB);
+      const double* cr = Cb + (size_t)gCb[row] * ldC + colC + lr;
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) acc[bj][r] = m ? 0.0 : cr[16 * bj];
+    }
+  }
+  __syncthreads();
+  if (row0 >= Np) return;
+#pragma unroll
+  for (int kk = 0; kk < NB / 4; ++kk) {
+    const int kq = 4 * kk + lq;
+    const double a = Ab[(size_t)(row0 + lr) * NB + kq];
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+      acc[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sB[kq][16 * bj + lr], acc[bj], 0, 0, 0);
+  }
+  double* Ob = Out + (size_t)b * Np * NB;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = row0 + mfma_drow(lq, r);
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) Ob[(size_t)row * NB + 16 * bj + lr] = acc[bj][r];
+  }
+}
+
+// Paired block step k, k+1 (columns [k0, k0+NB) and [k1, k1+NB), k1 = k0+NB):
+//   X_{k+2}[i][c] = mask(i) X_k[s1(s2(i))][c]
+//                   + A1[i] . X_k[p1][c] + P2[i] . X_k[s1(p2)][c]
+// with mask(i) = 0 if i is in the step-(k+1) pivot block or s2(i) in the step-k
+// pivot block; panel columns are replaced by A1 (step k) and P2 (step k+1).
+// Rank-2NB FP64 MFMA update: one pass over X for two Gauss-Jordan steps.
+__global__ __launch_bounds__(256) void k_gj_update2(const double* __restrict__ X, double* __restrict__ Out, int Np,
+                                                    int k0, const double* __restrict__ A1,
+                                                    const double* __restrict__ P2, const int* __restrict__ s1,
+                                                    const int* __restrict__ s2, const int* __restrict__ p1,
+                                                    const int* __restrict__ p2) {
+  constexpr int K2 = 2 * NB;
+  __shared__ double sB[K2][64];
+  const int tiles = (Np + 63) / 64;
+  const int I0 = (blockIdx.x / tiles) * 64;
+  const int J0 = (blockIdx.x % tiles) * 64;
+  const int b = blockIdx.y;
+  const int t = threadIdx.x;
+  const int wave = t >> 6, lane = t & 63, lr = lane & 15, lq = lane >> 4;
+  const int i0 = I0 + (wave >> 1) * 32;
+  const int j0 = J0 + (wave & 1) * 32;
+  const int k1 = k0 + NB;
+  const bool active = (i0 < Np) && (j0 < Np);
+  const double* Xb = X + (size_t)b * Np * Np;
+  double* Ob = Out + (size_t)b * Np * Np;
+  const double* A1b = A1 + (size_t)b * Np * NB;
+  const double* P2b = P2 + (size_t)b * Np * NB;
+  const int* s1b = s1 + (size_t)b * Np;
+  const int* s2b = s2 + (size_t)b * Np;
+
+  {  // stage the 2NB gathered B rows of the tile's 64 columns
+    const int k = t >> 2, part = t & 3;  // 64 rows x 4 parts of 16 columns
+    const int src_row = (k < NB) ? p1[b * NB + k] : s1b[p2[b * NB + (k - NB)]];
+    const int gj = J0 + part * 16;
+    const double2* br = reinterpret_cast<const double2*>(Xb + (size_t)src_row * Np + gj);
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      const double2 d = (gj < Np) ? br[h] : make_double2(0.0, 0.0);
+      sB[k][part * 16 + 2 * h] = d.x;
+      sB[k][part * 16 + 2 * h + 1] = d.y;
+    }
+  }
+  const bool pan1 = (j0 == k0), pan2 = (j0 == k1);
+  double4_t acc[2][2];
+  if (active && !pan1 && !pan2) {
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = i0 + 16 * bi + mfma_drow(lq, r);
+        const int m = s2b[row];
+        const bool zero = (row >= k1 && row < k1 + NB) || (m >= k0 && m < k0 + NB);
+        const double* xr = Xb + (size_t)s1b[m] * Np + j0 + lr;
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) acc[bi][bj][r] = zero ? 0.0 : xr[16 * bj];
+      }
+  }
+  __syncthreads();
+  if (!active) return;
+  if (pan1 || pan2) {
+    const double* Pb = pan1 ? A1b : P2b;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = i0 + 16 * bi + mfma_drow(lq, r);
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) Ob[(size_t)row * Np + j0 + 16 * bj + lr] = Pb[(size_t)row * NB + 16 * bj + lr];
+      }
+    return;
+  }
+  const int lj0 = j0 - J0;
+#pragma unroll
+  for (int kk = 0; kk < K2 / 4; ++kk) {
+    const int kq = 4 * kk + lq;
+    const double* Asrc = (kk < NB / 4) ? A1b : P2b;
+    const int kc = (kk < NB / 4) ? kq : kq - NB;
+    double a[2], bv[2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) a[bi] = Asrc[(size_t)(i0 + 16 * bi + lr) * NB + kc];
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) bv[bj] = sB[kq][lj0 + 16 * bj + lr];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], bv[bj], acc[bi][bj], 0, 0, 0);
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = i0 + 16 * bi + mfma_drow(lq, r);
+      double* orow = Ob + (size_t)row * Np + j0 + lr;
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) orow[16 * bj] = acc[bi][bj][r];
+    }
+}
+
 __global__ void k_unpermute(const double* __restrict__ in, double* __restrict__ out, int Np,
                             const int* __restrict__ pinv) {
   const int b = blockIdx.y;
@@ -414,40 +568,85 @@ int profile_end(double* total_ms, int* count, long long* total_batch) {
   return TICC_OK;
 }
 
+namespace {
+int launch_panel(const double* X, long long bstride, int ld, int col0, int batch, int Np, int k0, double* pout,
+                 int* src, int* piv, int* perm, int* pinv, hipStream_t stream) {
+  if (Np <= 256)
+    hipLaunchKernelGGL((k_gj_panel<256, 1>), dim3(batch), dim3(256), 0, stream, X, bstride, ld, col0, Np, k0, pout, src,
+                       piv, perm, pinv);
+  else if (Np <= 512)
+    hipLaunchKernelGGL((k_gj_panel<512, 1>), dim3(batch), dim3(512), 0, stream, X, bstride, ld, col0, Np, k0, pout, src,
+                       piv, perm, pinv);
+  else
+    hipLaunchKernelGGL((k_gj_panel<512, 2>), dim3(batch), dim3(512), 0, stream, X, bstride, ld, col0, Np, k0, pout, src,
+                       piv, perm, pinv);
+  TICC_CHECK_LAUNCH();
+  return TICC_OK;
+}
+}  // namespace
+
 int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result, int* perm, int* pinv,
                   double* pout, int* src, int* piv, hipStream_t stream) {
   if (Np % NB != 0 || Np > MAX_NP || Np <= 0 || batch <= 0)
     return fail(TICC_ERR_VALUE, "gj_invert_raw: invalid Np=%d batch=%d", Np, batch);
+  // scratch carve-up (sizes per GJ_POUT_DOUBLES / GJ_SRC_INTS / GJ_PIV_INTS)
+  const size_t pstride = (size_t)batch * Np * NB;
+  double* P1 = pout;
+  double* P2 = pout + pstride;
+  double* Z = pout + 2 * pstride;
+  double* A1 = pout + 3 * pstride;
+  int* s1 = src;
+  int* s2 = src + (size_t)batch * Np;
+  int* p1 = piv;
+  int* p2 = piv + batch * NB;
+  const long long n2 = (long long)Np * Np;
   double* cur = bufA;
   double* nxt = bufB;
   const int tiles = (Np + 63) / 64;
+  const int rtiles = (Np + 63) / 64;
   bool sample = false;
   if (g_prof.on) {
     std::lock_guard<std::mutex> lock(g_prof.mu);
     sample = g_prof.on && (g_prof.calls++ % g_prof.every == 0) &&
              (g_prof.used + (size_t)(Np / NB) <= g_prof.events.size() / 2);
   }
-  for (int k0 = 0; k0 < Np; k0 += NB) {
-    if (Np <= 256)
-      hipLaunchKernelGGL((k_gj_panel<256, 1>), dim3(batch), dim3(256), 0, stream, cur, Np, k0, pout, src, piv, perm,
-                         pinv);
-    else if (Np <= 512)
-      hipLaunchKernelGGL((k_gj_panel<512, 1>), dim3(batch), dim3(512), 0, stream, cur, Np, k0, pout, src, piv, perm,
-                         pinv);
-    else
-      hipLaunchKernelGGL((k_gj_panel<512, 2>), dim3(batch), dim3(512), 0, stream, cur, Np, k0, pout, src, piv, perm,
-                         pinv);
-    TICC_CHECK_LAUNCH();
+  const int steps = Np / NB;
+  for (int k = 0; k < steps;) {
+    const int k0 = k * NB;
+    TICC_RETURN_IF(launch_panel(cur, n2, Np, k0, batch, Np, k0, P1, s1, p1, perm, pinv, stream));
     size_t slot = 0;
-    if (sample) {
-      slot = g_prof.used++;
-      g_prof.batch[slot] = batch;
-      TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
+    if (k + 1 == steps) {  // trailing single step
+      if (sample) {
+        slot = g_prof.used++;
+        g_prof.batch[slot] = batch;
+        TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
+      }
+      hipLaunchKernelGGL(k_gj_update, dim3(tiles * tiles, batch), dim3(256), 0, stream, cur, nxt, Np, k0, P1, s1, p1);
+      TICC_CHECK_LAUNCH();
+      if (sample) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
+      k += 1;
+    } else {
+      const int k1 = k0 + NB;
+      // look-ahead panel columns of step k+1: Z = X_{k+1}[:, k1:k1+NB]
+      hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, stream, P1, cur, n2, Np, k1, s1, cur, n2, Np,
+                         k1, p1, k0, Np, Z);
+      TICC_CHECK_LAUNCH();
+      TICC_RETURN_IF(launch_panel(Z, (long long)Np * NB, NB, 0, batch, Np, k1, P2, s2, p2, perm, pinv, stream));
+      // combined step-k coefficients A1 = mask P1[s2] + P2 . P1[p2]
+      hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, stream, P2, P1, (long long)Np * NB, NB, 0, s2,
+                         P1, (long long)Np * NB, NB, 0, p2, k1, Np, A1);
+      TICC_CHECK_LAUNCH();
+      if (sample) {
+        slot = g_prof.used++;
+        g_prof.batch[slot] = 2 * batch;  // one launch performs two block steps
+        TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
+      }
+      hipLaunchKernelGGL(k_gj_update2, dim3(tiles * tiles, batch), dim3(256), 0, stream, cur, nxt, Np, k0, A1, P2, s1,
+                         s2, p1, p2);
+      TICC_CHECK_LAUNCH();
+      if (sample) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
+      k += 2;
     }
-    hipLaunchKernelGGL(k_gj_update, dim3(tiles * tiles, batch), dim3(256), 0, stream, cur, nxt, Np, k0, pout, src,
-                       piv);
-    TICC_CHECK_LAUNCH();
-    if (sample) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
     double* tmp = cur;
     cur = nxt;
     nxt = tmp;

@@ -253,9 +253,9 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
     Scratch sc{};
     sc.A0 = yv ? take(sizeof(double) * batch * n2) : 0;
     sc.A1 = yv ? take(sizeof(double) * batch * n2) : 0;
-    sc.P = take(sizeof(double) * batch * Np * GJ_NB);
-    sc.src = take(sizeof(int) * batch * Np);
-    sc.piv = take(sizeof(int) * batch * GJ_NB);
+    sc.P = take(sizeof(double) * GJ_POUT_PANELS * batch * Np * GJ_NB);
+    sc.src = take(sizeof(int) * GJ_SRC_MAPS * batch * Np);
+    sc.piv = take(sizeof(int) * GJ_SRC_MAPS * batch * GJ_NB);
     sc.perm = take(sizeof(int) * batch * Np);
     sc.pinv = take(sizeof(int) * batch * Np);
     sc.y1 = yv ? take(sizeof(double) * batch * Np) : 0;
