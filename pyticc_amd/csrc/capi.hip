@@ -145,7 +145,7 @@ int ticc_invert_batched(int n_batch, int n_channel, double* A, void* stream) {
   const size_t o0 = take(sizeof(double) * n_batch * n2), o1 = take(sizeof(double) * n_batch * n2);
   const size_t oP = take(sizeof(double) * ticc::GJ_POUT_PANELS * n_batch * Np * ticc::GJ_NB);
   const size_t oS = take(sizeof(int) * ticc::GJ_SRC_MAPS * n_batch * Np),
-               oV = take(sizeof(int) * ticc::GJ_SRC_MAPS * n_batch * ticc::GJ_NB);
+               oV = take(sizeof(int) * ticc::GJ_PIV_LISTS * n_batch * ticc::GJ_NB);
   const size_t oM = take(sizeof(int) * n_batch * Np), oI = take(sizeof(int) * n_batch * Np);
   void* base = nullptr;
   TICC_RETURN_IF(ticc::workspace(ticc::WS_INVERT, off, &base));

@@ -1,17 +1,24 @@
 // Batched blocked Gauss-Jordan inversion with partial pivoting (gfx950).
 //
-// A batch of padded Np x Np FP64 matrices is inverted in Np / NB block steps.
-// Step k0 (columns [k0, k0+NB)):
-//   1. k_gj_panel   (one workgroup per matrix): in-place Gauss-Jordan on the
-//      Np x NB column panel, pivot rows chosen by partial pivoting among the
-//      rows not yet used (LAPACK idamax order: largest |a|, first position on
-//      ties).  The panel lives in registers, up to 2 rows per thread.  Outputs the
-//      transformed panel in output-row order, the row source map of the step
-//      and the NB pivot rows.
-//   2. k_gj_update  (tiles x matrices): the rank-NB update
-//        Out[i][:] = Cin_i + Pout[i][:] . X[piv][:],  Cin_i = X[src(i)] (0 on
-//      the pivot block), panel columns replaced by Pout -- an FP64 MFMA GEMM
-//      with K = NB, reading X and writing the ping-pong partner buffer.
+// A batch of padded Np x Np FP64 matrices is inverted in Np / NB block steps
+// of NB columns, taken two at a time (k, k+1 -> one rank-2NB update):
+//   1. k_gj_panel (one workgroup per matrix): in-place Gauss-Jordan on the
+//      Np x NB column panel of step k, pivot rows chosen by partial pivoting
+//      among the rows not yet used (LAPACK idamax order: largest |a|, first
+//      position on ties).  The panel lives in registers, up to 2 rows per
+//      thread.  Outputs the transformed panel P_k in output-row order, the
+//      row source map s_k and the NB pivot rows p_k.
+//   2. k_panel_gemm: the columns of step k+1 after step k,
+//        Z[i] = mask_k(i) X[s_k(i)][cols_{k+1}] + P_k[i] . X[p_k][cols_{k+1}],
+//      then k_gj_panel on Z gives P_{k+1}, s_{k+1}, p_{k+1}, the combined row
+//      map and the 2NB rows of X that the pair multiplies.
+//   3. k_panel_gemm again: combined step-k coefficients
+//        A_k[i] = mask_{k+1}(i) P_k[s_{k+1}(i)] + P_{k+1}[i] . P_k[p_{k+1}].
+//   4. k_gj_update<2NB> (row bands x matrices): the rank-2NB update
+//        Out[i][:] = m(i) X[s_k(s_{k+1}(i))][:] + [A_k | P_{k+1}][i] . [X[p_k]; X[s_k(p_{k+1})]]
+//      -- one pass over X for two Gauss-Jordan steps, FP64 MFMA, reading X and
+//      writing the ping-pong partner buffer; the panel columns are replaced by
+//      the coefficients.  An odd last step uses the rank-NB form.
 // Row swaps compose into perm (original row at each position); the inverse of
 // the matrix is Out with its columns gathered through pinv = perm^-1, which the
 // consumers fuse into their own reads.
@@ -108,7 +115,10 @@ template <int THREADS, int RPT>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 256, THREADS / 256))) void k_gj_panel(const double* __restrict__ X, long long bstride, int ld, int col0, int Np, int k0,
                                                       double* __restrict__ Pout, int* __restrict__ src,
                                                       int* __restrict__ piv, int* __restrict__ perm,
-                                                      int* __restrict__ pinv) {
+                                                      int* __restrict__ pinv, double* __restrict__ PoutT,
+                                                      long long tstride, const int* __restrict__ prev_src,
+                                                      const int* __restrict__ prev_piv, int prev_k0,
+                                                      int* __restrict__ src2, int* __restrict__ brow) {
   constexpr int NW = THREADS / 64;
   __shared__ double s_vals[2][NW][NB];
   __shared__ double s_v[2][NW];
@@ -232,9 +242,25 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
 #pragma unroll
     for (int l = 0; l < NB / 2; ++l) op[l] = make_double2(v[q][2 * l], v[q][2 * l + 1]);
     src[(size_t)b * Np + p] = r;
-    if (p >= k0 && p < k0 + NB) piv[b * NB + (p - k0)] = r;
+    const bool blk = (p >= k0 && p < k0 + NB);
+    if (blk) piv[b * NB + (p - k0)] = r;
+    if (PoutT) {
+#pragma unroll
+      for (int l = 0; l < NB; ++l) PoutT[(size_t)b * tstride + (size_t)l * Np + p] = v[q][l];
+    }
+    if (src2) {  // row maps of the update that follows this panel
+      if (prev_src) {  // second step of a pair: compose with the first step's maps
+        const int s1r = prev_src[(size_t)b * Np + r];
+        src2[(size_t)b * Np + p] = (blk || (r >= prev_k0 && r < prev_k0 + NB)) ? -1 : s1r;
+        if (blk) brow[b * 2 * NB + NB + (p - k0)] = s1r;
+      } else {
+        src2[(size_t)b * Np + p] = blk ? -1 : r;
+        if (blk) brow[b * 2 * NB + (p - k0)] = r;
+      }
+    }
     pold[q] = (k0 == 0) ? r : perm[(size_t)b * Np + r];
   }
+  if (src2 && prev_src && t < NB) brow[b * 2 * NB + t] = prev_piv[b * NB + t];
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
@@ -244,125 +270,17 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
   }
 }
 
-// Update: 256 threads = 4 waves on a 64 x 64 output tile, each wave 32 x 32
-// (2 x 2 FP64 MFMA 16x16x4 blocks, K = NB).  The Pout rows and the NB pivot
-// rows of the tile are staged through LDS with coalesced loads; the gathered
-// input rows X[src(i)] are loaded straight into the accumulators before the
-// barrier.  Panel columns are overwritten with the transformed panel.
-__global__ __launch_bounds__(256) void k_gj_update(const double* __restrict__ X, double* __restrict__ Out, int Np,
-                                                   int k0, const double* __restrict__ Pout,
-                                                   const int* __restrict__ src, const int* __restrict__ piv) {
-  __shared__ double sA[64][NB + 1];
-  __shared__ double sB[NB][64];
-  const int tiles = (Np + 63) / 64;
-  const int I0 = (blockIdx.x / tiles) * 64;
-  const int J0 = (blockIdx.x % tiles) * 64;
-  const int b = blockIdx.y;
-  const int t = threadIdx.x;
-  const int wave = t >> 6;
-  const int lane = t & 63;
-  const int i0 = I0 + (wave >> 1) * 32;
-  const int j0 = J0 + (wave & 1) * 32;
-  const bool active = (i0 < Np) && (j0 < Np);
-  const bool panel = (j0 == k0);
-
-  const double* Xb = X + (size_t)b * Np * Np;
-  double* Ob = Out + (size_t)b * Np * Np;
-  const double* Pb = Pout + (size_t)b * Np * NB;
-  const int* srcb = src + (size_t)b * Np;
-  const int* pivb = piv + b * NB;
-  const int lr = lane & 15;
-  const int lq = lane >> 4;
-
-  // stage A = Pout[I0:I0+64][0:NB] and B = X[piv][J0:J0+64]
-  {
-    const int row = t >> 2, part = t & 3;
-    const int gi = I0 + row;
-    double2 a[4];
-#pragma unroll
-    for (int h = 0; h < 4; ++h) a[h] = (gi < Np) ? reinterpret_cast<const double2*>(Pb + (size_t)gi * NB + part * 8)[h] : make_double2(0.0, 0.0);
-#pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      sA[row][part * 8 + 2 * h] = a[h].x;
-      sA[row][part * 8 + 2 * h + 1] = a[h].y;
-    }
-    const int k = t >> 3, seg = t & 7;
-    const int gj = J0 + seg * 8;
-    const double* brow = Xb + (size_t)pivb[k] * Np + gj;
-    double2 bb[4];
-#pragma unroll
-    for (int h = 0; h < 4; ++h) bb[h] = (gj < Np) ? reinterpret_cast<const double2*>(brow)[h] : make_double2(0.0, 0.0);
-#pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      sB[k][seg * 8 + 2 * h] = bb[h].x;
-      sB[k][seg * 8 + 2 * h + 1] = bb[h].y;
-    }
-  }
-
-  double4_t acc[2][2];
-  if (active && !panel) {
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = i0 + 16 * bi + mfma_drow(lq, r);
-        const bool pivblk = (row >= k0) && (row < k0 + NB);
-        const double* xr = Xb + (size_t)srcb[row] * Np + j0 + lr;
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj) acc[bi][bj][r] = pivblk ? 0.0 : xr[16 * bj];
-      }
-  }
-  __syncthreads();
-  if (!active) return;
-  const int li0 = i0 - I0;
-  const int lj0 = j0 - J0;
-  if (panel) {
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-          const int lrow = li0 + 16 * bi + mfma_drow(lq, r);
-          const int col = 16 * bj + lr;
-          Ob[(size_t)(I0 + lrow) * Np + k0 + col] = sA[lrow][col];
-        }
-    return;
-  }
-#pragma unroll
-  for (int kk = 0; kk < NB / 4; ++kk) {
-    const int kq = 4 * kk + lq;
-    double a[2], bv[2];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi) a[bi] = sA[li0 + 16 * bi + lr][kq];
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj) bv[bj] = sB[kq][lj0 + 16 * bj + lr];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], bv[bj], acc[bi][bj], 0, 0, 0);
-  }
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = i0 + 16 * bi + mfma_drow(lq, r);
-      double* orow = Ob + (size_t)row * Np + j0 + lr;
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj) orow[16 * bj] = acc[bi][bj][r];
-    }
-}
-
-// Small panel GEMM (FP64 MFMA) used by the paired block step:
+// Small panel GEMM (FP64 MFMA) of the paired block step:
 //   Out[i][0:NB] = mask(i) C[gC(i)][colC : colC+NB] + A[i][0:NB] . B[gB(r)][colB : colB+NB],
 //   mask(i) = 0 for i in [mlo, mlo+NB), 1 otherwise.
-// One workgroup (4 waves) per 64 rows of one matrix, each wave 16 rows x NB.
+// Out is row-major (Np x NB) or, with `ostride` > 0, transposed (NB rows of
+// length Np, matrix stride ostride).  One workgroup (4 waves) per 64 rows of one
+// matrix, each wave 16 rows x NB.
 __global__ __launch_bounds__(256) void k_panel_gemm(const double* __restrict__ A, const double* __restrict__ C,
                                                     long long cb, int ldC, int colC, const int* __restrict__ gC,
                                                     const double* __restrict__ Bm, long long bb, int ldB, int colB,
                                                     const int* __restrict__ gB, int mlo, int Np,
-                                                    double* __restrict__ Out) {
+                                                    double* __restrict__ Out, long long ostride) {
   __shared__ double sB[NB][NB + 1];
   const int b = blockIdx.y;
   const int t = threadIdx.x;
@@ -379,19 +297,21 @@ __global__ __launch_bounds__(256) void k_panel_gemm(const double* __restrict__ A
     for (int h = 0; h < 4; ++h) sB[r][seg * 4 + h] = br[h];
   }
   const int row0 = blockIdx.x * 64 + wave * 16;
+  const bool active = row0 < Np;
   double4_t acc[2];
-  if (row0 < Np) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = row0 + mfma_drow(lq, r);
-      const bool m = (row >= mlo) && (row < mlo + NB);
-      const double* cr = Cb + (size_t)gCb[row] * ldC + colC + lr;
+  for (int r = 0; r < 4; ++r) {
+    const int row = active ? row0 + mfma_drow(lq, r) : 0;
+    const bool m = (row >= mlo) && (row < mlo + NB);
+    const double* cr = Cb + (size_t)gCb[row] * ldC + colC + lr;
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj) acc[bj][r] = m ? 0.0 : cr[16 * bj];
+    for (int bj = 0; bj < 2; ++bj) {
+      const double cv = cr[16 * bj];
+      acc[bj][r] = m ? 0.0 : cv;
     }
   }
   __syncthreads();
-  if (row0 >= Np) return;
+  if (!active) return;
 #pragma unroll
   for (int kk = 0; kk < NB / 4; ++kk) {
     const int kq = 4 * kk + lq;
@@ -400,112 +320,195 @@ __global__ __launch_bounds__(256) void k_panel_gemm(const double* __restrict__ A
     for (int bj = 0; bj < 2; ++bj)
       acc[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sB[kq][16 * bj + lr], acc[bj], 0, 0, 0);
   }
-  double* Ob = Out + (size_t)b * Np * NB;
+  if (ostride > 0) {
+    double* Ob = Out + (size_t)b * ostride;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int row = row0 + mfma_drow(lq, r);
+    for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) Ob[(size_t)row * NB + 16 * bj + lr] = acc[bj][r];
+      for (int bj = 0; bj < 2; ++bj) Ob[(size_t)(16 * bj + lr) * Np + row0 + mfma_drow(lq, r)] = acc[bj][r];
+  } else {
+    double* Ob = Out + (size_t)b * Np * NB;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = row0 + mfma_drow(lq, r);
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) Ob[(size_t)row * NB + 16 * bj + lr] = acc[bj][r];
+    }
   }
 }
 
-// Paired block step k, k+1 (columns [k0, k0+NB) and [k1, k1+NB), k1 = k0+NB):
-//   X_{k+2}[i][c] = mask(i) X_k[s1(s2(i))][c]
-//                   + A1[i] . X_k[p1][c] + P2[i] . X_k[s1(p2)][c]
-// with mask(i) = 0 if i is in the step-(k+1) pivot block or s2(i) in the step-k
-// pivot block; panel columns are replaced by A1 (step k) and P2 (step k+1).
-// Rank-2NB FP64 MFMA update: one pass over X for two Gauss-Jordan steps.
-__global__ __launch_bounds__(256) void k_gj_update2(const double* __restrict__ X, double* __restrict__ Out, int Np,
-                                                    int k0, const double* __restrict__ A1,
-                                                    const double* __restrict__ P2, const int* __restrict__ s1,
-                                                    const int* __restrict__ s2, const int* __restrict__ p1,
-                                                    const int* __restrict__ p2) {
-  constexpr int K2 = 2 * NB;
-  __shared__ double sB[K2][64];
+// Raw buffer loads: an offset past the end of the buffer reads 0 without a
+// memory access, so masked rows need no branch (a branch around a load makes
+// the compiler wait for that load at the join).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+constexpr int kOutOfRange = 0x7ff00000;
+__device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ double2 buffer_load_f64x2(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+// Rank-KB update (KB = NB for one step, 2 NB for a pair of steps):
+//   Out[i][:] = (src2(i) < 0 ? 0 : X[src2(i)][:]) + AT[:, i]^T . X[brow][:],
+// the columns [k0, k0+KB) replaced by AT^T.  AT holds the KB coefficient columns
+// transposed (KB rows of length Np, matrix stride 2 NB Np) so an MFMA A fragment
+// is 16 consecutive doubles.  One workgroup (4 waves, 32 x 32 each) walks a
+// 64-row band across a group of 64-column tiles: the band's A fragments stay in
+// registers, and the next tile's C rows (into the idle accumulator set) and B
+// rows (into the idle LDS buffer) are loaded while the current tile's MFMAs
+// and stores run.
+template <int KB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_update(
+    const double* __restrict__ X, double* __restrict__ Out, int Np, int k0, const double* __restrict__ AT,
+    const int* __restrict__ src2, const int* __restrict__ brow, int ngroups) {
+  constexpr int KK = KB / 4;
+  __shared__ double sB[2][KB][64];
   const int tiles = (Np + 63) / 64;
-  const int I0 = (blockIdx.x / tiles) * 64;
-  const int J0 = (blockIdx.x % tiles) * 64;
   const int b = blockIdx.y;
+  const int band = blockIdx.x / ngroups, grp = blockIdx.x % ngroups;
+  const int jt0 = (grp * tiles) / ngroups, jt1 = ((grp + 1) * tiles) / ngroups;
+  if (jt0 >= jt1) return;
   const int t = threadIdx.x;
   const int wave = t >> 6, lane = t & 63, lr = lane & 15, lq = lane >> 4;
-  const int i0 = I0 + (wave >> 1) * 32;
-  const int j0 = J0 + (wave & 1) * 32;
-  const int k1 = k0 + NB;
-  const bool active = (i0 < Np) && (j0 < Np);
+  const int i0 = band * 64 + (wave >> 1) * 32;
+  const int jw = (wave & 1) * 32;
+  const bool rows_ok = i0 < Np;
   const double* Xb = X + (size_t)b * Np * Np;
   double* Ob = Out + (size_t)b * Np * Np;
-  const double* A1b = A1 + (size_t)b * Np * NB;
-  const double* P2b = P2 + (size_t)b * Np * NB;
-  const int* s1b = s1 + (size_t)b * Np;
-  const int* s2b = s2 + (size_t)b * Np;
+  const auto rX = make_rsrc(Xb, (unsigned)(sizeof(double) * Np * Np));
+  const auto rA = make_rsrc(AT + (size_t)b * 2 * NB * Np, (unsigned)(sizeof(double) * KB * Np));
+  const auto rS = make_rsrc(src2 + (size_t)b * Np, (unsigned)(sizeof(int) * Np));
+  const int* brb = brow + b * 2 * NB;
 
-  {  // stage the 2NB gathered B rows of the tile's 64 columns
-    const int k = t >> 2, part = t & 3;  // 64 rows x 4 parts of 16 columns
-    const int src_row = (k < NB) ? p1[b * NB + k] : s1b[p2[b * NB + (k - NB)]];
-    const int gj = J0 + part * 16;
-    const double2* br = reinterpret_cast<const double2*>(Xb + (size_t)src_row * Np + gj);
+  double a[2][KK];
+  int sr[2][4];
 #pragma unroll
-    for (int h = 0; h < 8; ++h) {
-      const double2 d = (gj < Np) ? br[h] : make_double2(0.0, 0.0);
-      sB[k][part * 16 + 2 * h] = d.x;
-      sB[k][part * 16 + 2 * h + 1] = d.y;
-    }
-  }
-  const bool pan1 = (j0 == k0), pan2 = (j0 == k1);
-  double4_t acc[2][2];
-  if (active && !pan1 && !pan2) {
+  for (int kk = 0; kk < KK; ++kk)
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = i0 + 16 * bi + mfma_drow(lq, r);
-        const int m = s2b[row];
-        const bool zero = (row >= k1 && row < k1 + NB) || (m >= k0 && m < k0 + NB);
-        const double* xr = Xb + (size_t)s1b[m] * Np + j0 + lr;
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj) acc[bi][bj][r] = zero ? 0.0 : xr[16 * bj];
-      }
-  }
-  __syncthreads();
-  if (!active) return;
-  if (pan1 || pan2) {
-    const double* Pb = pan1 ? A1b : P2b;
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = i0 + 16 * bi + mfma_drow(lq, r);
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj) Ob[(size_t)row * Np + j0 + 16 * bj + lr] = Pb[(size_t)row * NB + 16 * bj + lr];
-      }
-    return;
-  }
-  const int lj0 = j0 - J0;
-#pragma unroll
-  for (int kk = 0; kk < K2 / 4; ++kk) {
-    const int kq = 4 * kk + lq;
-    const double* Asrc = (kk < NB / 4) ? A1b : P2b;
-    const int kc = (kk < NB / 4) ? kq : kq - NB;
-    double a[2], bv[2];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi) a[bi] = Asrc[(size_t)(i0 + 16 * bi + lr) * NB + kc];
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj) bv[bj] = sB[kq][lj0 + 16 * bj + lr];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], bv[bj], acc[bi][bj], 0, 0, 0);
-  }
+      a[bi][kk] = buffer_load_f64(rA, rows_ok ? 8 * ((4 * kk + lq) * Np + i0 + 16 * bi + lr) : kOutOfRange);
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = i0 + 16 * bi + mfma_drow(lq, r);
-      double* orow = Ob + (size_t)row * Np + j0 + lr;
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj) orow[16 * bj] = acc[bi][bj][r];
+      const int v = (int)__builtin_amdgcn_raw_buffer_load_b32(
+          rS, rows_ok ? 4 * (i0 + 16 * bi + mfma_drow(lq, r)) : kOutOfRange, 0, 0);
+      sr[bi][r] = rows_ok ? v : -1;
     }
+  const int half = t >> 5, c2 = (t & 31) * 2;
+  int bro[KB / 8];
+#pragma unroll
+  for (int h = 0; h < KB / 8; ++h) bro[h] = brb[h * 8 + half];
+
+  // 0: update tile, 1: coefficient (panel) columns, 2: outside the matrix
+  auto kind = [&](int J) {
+    const int j0 = J * 64 + jw;
+    if (!rows_ok || j0 >= Np) return 2;
+    return (j0 >= k0 && j0 < k0 + KB) ? 1 : 0;
+  };
+  auto load_c = [&](int J, double4_t(&c)[2][2]) {
+    const int j0 = J * 64 + jw;
+    const bool upd = kind(J) == 0;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int s = sr[bi][r];
+        const int base = (upd && s >= 0) ? 8 * (s * Np + j0 + lr) : kOutOfRange;
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) c[bi][bj][r] = buffer_load_f64(rX, base + 128 * bj);
+      }
+  };
+  double2 bst[KB / 8];
+  auto load_b = [&](int J) {
+    const bool in = (J * 64 + c2) < Np;
+#pragma unroll
+    for (int h = 0; h < KB / 8; ++h)
+      bst[h] = buffer_load_f64x2(rX, in ? 8 * (bro[h] * Np + J * 64 + c2) : kOutOfRange);
+  };
+  // B tile rows are stored with bit 4 of the column flipped on odd rows, so the
+  // four rows an MFMA B fragment reads fall in different LDS bank halves.
+  auto store_b = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < KB / 8; ++h) {
+      const int k = h * 8 + half;
+      *reinterpret_cast<double2*>(&sB[buf][k][c2 ^ ((k & 1) << 4)]) = bst[h];
+    }
+  };
+  auto compute_store = [&](int J, int buf, double4_t(&acc)[2][2]) {
+    const int kd = kind(J);
+    const int j0 = J * 64 + jw;
+    if (kd == 0) {
+#pragma unroll
+      for (int kk = 0; kk < KK; ++kk) {
+        const int kq = 4 * kk + lq;
+        const int sw = (kq & 1) << 4;
+        double bv[2];
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) bv[bj] = sB[buf][kq][(jw + 16 * bj + lr) ^ sw];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+          for (int bj = 0; bj < 2; ++bj)
+            acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi][kk], bv[bj], acc[bi][bj], 0, 0, 0);
+      }
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double* orow = Ob + (size_t)(i0 + 16 * bi + mfma_drow(lq, r)) * Np + j0 + lr;
+#pragma unroll
+          for (int bj = 0; bj < 2; ++bj) orow[16 * bj] = acc[bi][bj][r];
+        }
+    } else if (kd == 1) {  // coefficient columns: Out[i][j0 + c] = AT[j0 - k0 + c][i]
+      const int kb = j0 - k0;
+      double pv[2][4][2];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int bj = 0; bj < 2; ++bj)
+            pv[bi][r][bj] = buffer_load_f64(rA, 8 * ((kb + 16 * bj + lr) * Np + i0 + 16 * bi + mfma_drow(lq, r)));
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double* orow = Ob + (size_t)(i0 + 16 * bi + mfma_drow(lq, r)) * Np + j0 + lr;
+#pragma unroll
+          for (int bj = 0; bj < 2; ++bj) orow[16 * bj] = pv[bi][r][bj];
+        }
+    }
+  };
+
+  double4_t accA[2][2], accB[2][2];
+  load_b(jt0);
+  load_c(jt0, accA);
+  store_b(0);
+  __syncthreads();
+  for (int J = jt0;; J += 2) {
+    // even tile: accumulators A, LDS buffer 0; prefetch into B / buffer 1
+    if (J + 1 < jt1) {
+      load_b(J + 1);
+      load_c(J + 1, accB);
+    }
+    compute_store(J, 0, accA);
+    if (J + 1 >= jt1) break;
+    store_b(1);
+    __syncthreads();
+    // odd tile: accumulators B, LDS buffer 1; prefetch into A / buffer 0
+    if (J + 2 < jt1) {
+      load_b(J + 2);
+      load_c(J + 2, accA);
+    }
+    compute_store(J + 1, 1, accB);
+    if (J + 2 >= jt1) break;
+    store_b(0);
+    __syncthreads();
+  }
 }
 
 __global__ void k_unpermute(const double* __restrict__ in, double* __restrict__ out, int Np,
@@ -569,18 +572,50 @@ int profile_end(double* total_ms, int* count, long long* total_batch) {
 }
 
 namespace {
+struct PanelMaps {  // optional outputs of a panel launch
+  double* PoutT = nullptr;
+  long long tstride = 0;
+  const int* prev_src = nullptr;
+  const int* prev_piv = nullptr;
+  int prev_k0 = 0;
+  int* src2 = nullptr;
+  int* brow = nullptr;
+};
+
 int launch_panel(const double* X, long long bstride, int ld, int col0, int batch, int Np, int k0, double* pout,
-                 int* src, int* piv, int* perm, int* pinv, hipStream_t stream) {
+                 int* src, int* piv, int* perm, int* pinv, const PanelMaps& m, hipStream_t stream) {
+#define TICC_PANEL_ARGS                                                                                          \
+  X, bstride, ld, col0, Np, k0, pout, src, piv, perm, pinv, m.PoutT, m.tstride, m.prev_src, m.prev_piv, m.prev_k0, \
+      m.src2, m.brow
   if (Np <= 256)
-    hipLaunchKernelGGL((k_gj_panel<256, 1>), dim3(batch), dim3(256), 0, stream, X, bstride, ld, col0, Np, k0, pout, src,
-                       piv, perm, pinv);
+    hipLaunchKernelGGL((k_gj_panel<256, 1>), dim3(batch), dim3(256), 0, stream, TICC_PANEL_ARGS);
   else if (Np <= 512)
-    hipLaunchKernelGGL((k_gj_panel<512, 1>), dim3(batch), dim3(512), 0, stream, X, bstride, ld, col0, Np, k0, pout, src,
-                       piv, perm, pinv);
+    hipLaunchKernelGGL((k_gj_panel<512, 1>), dim3(batch), dim3(512), 0, stream, TICC_PANEL_ARGS);
   else
-    hipLaunchKernelGGL((k_gj_panel<512, 2>), dim3(batch), dim3(512), 0, stream, X, bstride, ld, col0, Np, k0, pout, src,
-                       piv, perm, pinv);
+    hipLaunchKernelGGL((k_gj_panel<512, 2>), dim3(batch), dim3(512), 0, stream, TICC_PANEL_ARGS);
+#undef TICC_PANEL_ARGS
   TICC_CHECK_LAUNCH();
+  return TICC_OK;
+}
+
+template <int KB>
+int launch_update(const double* cur, double* nxt, int batch, int Np, int k0, const double* AT, const int* src2,
+                  const int* brow, bool sample, hipStream_t stream) {
+  const int tiles = (Np + 63) / 64;
+  // enough workgroups to fill the chip twice over; each walks tiles/ngroups tiles
+  int ngroups = (2048 + tiles * batch - 1) / (tiles * batch);
+  if (ngroups > tiles) ngroups = tiles;
+  if (ngroups < 1) ngroups = 1;
+  size_t slot = 0;
+  if (sample) {
+    slot = g_prof.used++;
+    g_prof.batch[slot] = batch * (KB / NB);  // block steps performed by this launch
+    TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
+  }
+  hipLaunchKernelGGL((k_gj_update<KB>), dim3(tiles * ngroups, batch), dim3(256), 0, stream, cur, nxt, Np, k0, AT,
+                     src2, brow, ngroups);
+  TICC_CHECK_LAUNCH();
+  if (sample) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
   return TICC_OK;
 }
 }  // namespace
@@ -589,20 +624,23 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
                   double* pout, int* src, int* piv, hipStream_t stream) {
   if (Np % NB != 0 || Np > MAX_NP || Np <= 0 || batch <= 0)
     return fail(TICC_ERR_VALUE, "gj_invert_raw: invalid Np=%d batch=%d", Np, batch);
-  // scratch carve-up (sizes per GJ_POUT_DOUBLES / GJ_SRC_INTS / GJ_PIV_INTS)
-  const size_t pstride = (size_t)batch * Np * NB;
+  // scratch carve-up: GJ_POUT_PANELS, GJ_SRC_MAPS and GJ_PIV_LISTS per matrix
+  const size_t ps = (size_t)batch * Np * NB;
   double* P1 = pout;
-  double* P2 = pout + pstride;
-  double* Z = pout + 2 * pstride;
-  double* A1 = pout + 3 * pstride;
+  double* P2 = pout + ps;
+  double* Z = pout + 2 * ps;
+  double* AT = pout + 3 * ps;  // 2 NB x Np per matrix
+  const long long at_stride = 2LL * NB * Np;
   int* s1 = src;
   int* s2 = src + (size_t)batch * Np;
+  int* src2 = src + 2 * (size_t)batch * Np;
   int* p1 = piv;
   int* p2 = piv + batch * NB;
+  int* brow = piv + 2 * batch * NB;  // 2 NB per matrix
   const long long n2 = (long long)Np * Np;
+  const long long pn = (long long)Np * NB;
   double* cur = bufA;
   double* nxt = bufB;
-  const int tiles = (Np + 63) / 64;
   const int rtiles = (Np + 63) / 64;
   bool sample = false;
   if (g_prof.on) {
@@ -613,38 +651,36 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
   const int steps = Np / NB;
   for (int k = 0; k < steps;) {
     const int k0 = k * NB;
-    TICC_RETURN_IF(launch_panel(cur, n2, Np, k0, batch, Np, k0, P1, s1, p1, perm, pinv, stream));
-    size_t slot = 0;
-    if (k + 1 == steps) {  // trailing single step
-      if (sample) {
-        slot = g_prof.used++;
-        g_prof.batch[slot] = batch;
-        TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
-      }
-      hipLaunchKernelGGL(k_gj_update, dim3(tiles * tiles, batch), dim3(256), 0, stream, cur, nxt, Np, k0, P1, s1, p1);
-      TICC_CHECK_LAUNCH();
-      if (sample) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
+    if (k + 1 == steps) {  // trailing single step: rank-NB update
+      PanelMaps m;
+      m.PoutT = AT;
+      m.tstride = at_stride;
+      m.src2 = src2;
+      m.brow = brow;
+      TICC_RETURN_IF(launch_panel(cur, n2, Np, k0, batch, Np, k0, P1, s1, p1, perm, pinv, m, stream));
+      TICC_RETURN_IF(launch_update<NB>(cur, nxt, batch, Np, k0, AT, src2, brow, sample, stream));
       k += 1;
     } else {
       const int k1 = k0 + NB;
-      // look-ahead panel columns of step k+1: Z = X_{k+1}[:, k1:k1+NB]
+      TICC_RETURN_IF(launch_panel(cur, n2, Np, k0, batch, Np, k0, P1, s1, p1, perm, pinv, PanelMaps{}, stream));
+      // look-ahead columns of step k+1: Z = X_{k+1}[:, k1:k1+NB]
       hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, stream, P1, cur, n2, Np, k1, s1, cur, n2, Np,
-                         k1, p1, k0, Np, Z);
+                         k1, p1, k0, Np, Z, 0LL);
       TICC_CHECK_LAUNCH();
-      TICC_RETURN_IF(launch_panel(Z, (long long)Np * NB, NB, 0, batch, Np, k1, P2, s2, p2, perm, pinv, stream));
-      // combined step-k coefficients A1 = mask P1[s2] + P2 . P1[p2]
-      hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, stream, P2, P1, (long long)Np * NB, NB, 0, s2,
-                         P1, (long long)Np * NB, NB, 0, p2, k1, Np, A1);
+      PanelMaps m;
+      m.PoutT = AT + (size_t)NB * Np;
+      m.tstride = at_stride;
+      m.prev_src = s1;
+      m.prev_piv = p1;
+      m.prev_k0 = k0;
+      m.src2 = src2;
+      m.brow = brow;
+      TICC_RETURN_IF(launch_panel(Z, pn, NB, 0, batch, Np, k1, P2, s2, p2, perm, pinv, m, stream));
+      // combined step-k coefficients A1 = mask P1[s2] + P2 . P1[p2], stored transposed in AT[0:NB]
+      hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, stream, P2, P1, pn, NB, 0, s2, P1, pn, NB, 0,
+                         p2, k1, Np, AT, at_stride);
       TICC_CHECK_LAUNCH();
-      if (sample) {
-        slot = g_prof.used++;
-        g_prof.batch[slot] = 2 * batch;  // one launch performs two block steps
-        TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
-      }
-      hipLaunchKernelGGL(k_gj_update2, dim3(tiles * tiles, batch), dim3(256), 0, stream, cur, nxt, Np, k0, A1, P2, s1,
-                         s2, p1, p2);
-      TICC_CHECK_LAUNCH();
-      if (sample) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
+      TICC_RETURN_IF(launch_update<2 * NB>(cur, nxt, batch, Np, k0, AT, src2, brow, sample, stream));
       k += 2;
     }
     double* tmp = cur;

@@ -255,7 +255,7 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
     sc.A1 = yv ? take(sizeof(double) * batch * n2) : 0;
     sc.P = take(sizeof(double) * GJ_POUT_PANELS * batch * Np * GJ_NB);
     sc.src = take(sizeof(int) * GJ_SRC_MAPS * batch * Np);
-    sc.piv = take(sizeof(int) * GJ_SRC_MAPS * batch * GJ_NB);
+    sc.piv = take(sizeof(int) * GJ_PIV_LISTS * batch * GJ_NB);
     sc.perm = take(sizeof(int) * batch * Np);
     sc.pinv = take(sizeof(int) * batch * Np);
     sc.y1 = yv ? take(sizeof(double) * batch * Np) : 0;

@@ -210,7 +210,7 @@ int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, con
   const size_t oNc = take(sizeof(int) * nE);
   const size_t oP = take(sizeof(double) * GJ_POUT_PANELS * EC * Np * GJ_NB);
   const size_t oSrc = take(sizeof(int) * GJ_SRC_MAPS * EC * Np);
-  const size_t oPiv = take(sizeof(int) * GJ_SRC_MAPS * EC * GJ_NB);
+  const size_t oPiv = take(sizeof(int) * GJ_PIV_LISTS * EC * GJ_NB);
   const size_t oPerm = take(sizeof(int) * EC * Np);
   const size_t oPinv = take(sizeof(int) * EC * Np);
   const size_t oFlag = take(sizeof(int));

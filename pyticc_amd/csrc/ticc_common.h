@@ -27,11 +27,12 @@ int aux_events(int n, hipEvent_t* out);
 // every internal square matrix (Np = round_up(N, GJ_NB)).
 constexpr int GJ_NB = 32;
 constexpr int MAX_NP = 1024;
-// Scratch of gj_invert_raw per matrix: pout holds 4 Np x NB panels (two
-// transformed panels, the look-ahead columns and the combined coefficients),
-// src 2 Np row maps, piv 2 NB pivot lists.
-constexpr int GJ_POUT_PANELS = 4;
-constexpr int GJ_SRC_MAPS = 2;
+// Scratch of gj_invert_raw per matrix: pout holds 5 Np x NB panels (two
+// transformed panels, the look-ahead columns and the 2 NB transposed update
+// coefficients), src 3 Np row maps, piv 4 NB-entry row lists.
+constexpr int GJ_POUT_PANELS = 5;
+constexpr int GJ_SRC_MAPS = 3;
+constexpr int GJ_PIV_LISTS = 4;
 
 inline int round_up(int n, int q) { return (n + q - 1) / q * q; }
 
