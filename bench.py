@@ -8,14 +8,21 @@ basis rows resident in HBM: W(R) contraction at every radial point, the FP64
 LDMD sweep of all 64 energies, and the fused BF->SF + K->S matching.
 
 value = energies x sectors / seconds per step (energy.R-steps/s).
-roofline: the rank-32 Gauss-Jordan update kernel (the sweep's dominant
-kernel), timed live with HIP events; algorithmic FP64 flop per launch =
-2 N^2 * 32 * (matrices in the launch) (4N^3 per energy.R-step over 2 N/32 launches).
+roofline: the Gauss-Jordan update kernel k_gj_update (rank 64 = two block
+steps per launch; the sweep's dominant kernel), timed live with HIP events on
+the stream it runs on.  Per launch on m matrices: algorithmic bytes
+m (2 Np^2 + 64 Np) 8 (read X, write the partner buffer, read the coefficient
+columns) and FP64 flop 2 Np^2 64 m -- 8 flop/byte, below the FP64 ridge point,
+so the bound is HBM; the MFMA rate is reported beside it.  traffic: HBM bytes
+per 64-matrix launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+summary (profiles/), when it was measured on this workload.
 cpu_baseline: the NumPy oracle (a restatement of the reference algorithm) on
 the same system, all 64 energies over the first sectors until >= 10 s of
 work, on this host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu-baseline]
+       [--max-sectors M]   (first M sectors only: a short run for counter profiles;
+                            the JSON line then names the reduced workload)
 """
 
 from __future__ import annotations
@@ -76,6 +83,19 @@ def cpu_baseline(workload, grid, min_seconds: float = 10.0, max_sectors: int = 6
     }
 
 
+def _traffic(n_channel: int, n_energy: int):
+    """HBM bytes per rank-64 update launch from the committed counter summary, if it matches."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_gj_update_rotor_rotor_N800_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("n_channel") != n_channel or rec.get("matrices") != n_energy:
+        return None
+    return rec.get("hbm_bytes_per_launch")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +103,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-every", type=int, default=8, help="time every k-th batched inversion's update launches")
+    ap.add_argument("--max-sectors", type=int, default=0, help="propagate only the first M sectors (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (requested --gpus {args.gpus}); "
@@ -110,12 +131,13 @@ def main() -> int:
     dev = torch.device("cuda", 0)
     ham.W_points(grid.radial_points[:1], dev)  # PES grid and basis rows resident in HBM
     torch.cuda.synchronize()
-    N, nE, nS = w.system.n_channel, w.energies.size, len(grid.sectors)
+    sectors = grid.sectors[: args.max_sectors] if args.max_sectors > 0 else grid.sectors
+    N, nE, nS = w.system.n_channel, w.energies.size, len(sectors)
     _log(f"setup {time.perf_counter() - t0:.1f} s: N={N}, energies={nE}, sectors={nS}")
 
     for i in range(args.warmup):
         t = time.perf_counter()
-        solve_device(ham, w.energies, grid.sectors, prop, B, L)
+        solve_device(ham, w.energies, sectors, prop, B, L)
         torch.cuda.synchronize()
         _log(f"warmup {i}: {time.perf_counter() - t:.2f} s")
 
@@ -124,7 +146,7 @@ def main() -> int:
     torch.cuda.synchronize()
     t = time.perf_counter()
     for i in range(args.steps):
-        Y, S_re, S_im, n_open = solve_device(ham, w.energies, grid.sectors, prop, B, L)
+        Y, S_re, S_im, n_open = solve_device(ham, w.energies, sectors, prop, B, L)
         _log(f"step {i} enqueued/done at {time.perf_counter() - t:.2f} s")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t
@@ -137,14 +159,19 @@ def main() -> int:
     value = nE * nS / (elapsed / args.steps)
 
     Np = (N + 31) // 32 * 32
-    batch_per_launch = float(nbat[0]) / max(1, int(cnt[0]))
-    flop_per_launch = 2.0 * Np * Np * 32 * batch_per_launch
+    steps_x_matrices = float(nbat[0]) / max(1, int(cnt[0]))  # block steps x matrices per timed launch
+    matrices = steps_x_matrices / 2.0                          # rank-64 launches: two block steps
     avg_ms = float(tot_ms[0]) / max(1, int(cnt[0]))
-    achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if cnt[0] else None
+    flop_per_launch = 2.0 * Np * Np * 32 * steps_x_matrices
+    # algorithmic HBM bytes: read X, write the partner buffer, read the 64 coefficient columns
+    bytes_per_launch = matrices * (2.0 * Np * Np * 8 + 64.0 * Np * 8)
+    achieved_gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if cnt[0] else None
+    achieved_tf = flop_per_launch / (avg_ms * 1e-3) / 1e12 if cnt[0] else None
     S = unpack_smat(S_re, S_im, n_open)
     defect = max(float(np.max(np.abs(s.conj().T @ s - np.eye(s.shape[0])))) for s in S if s.size)
     _log(f"update kernel: {int(cnt[0])} timed launches, avg {avg_ms * 1e3:.1f} us; unitarity defect {defect:.2e}")
 
+    traffic = _traffic(N, nE)
     cpu = None
     if not args.no_cpu_baseline:
         t = time.perf_counter()
@@ -165,24 +192,28 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic (seeded Tang-Toennies rotor-rotor PES, rigid rotors)",
         "config": {
-            "workload": "rotor_rotor exact CC: N=800 channels, 64 energies 10.87-1000.87 cm^-1, 2000 LDMD sectors 4-40 bohr, "
-            "15x15x12 angular grid; step = contraction + FP64 LDMD sweep + matching",
+            "workload": f"rotor_rotor exact CC: N={N} channels, {nE} energies 10.87-1000.87 cm^-1, {nS} LDMD sectors "
+            f"4-{sectors[-1].radial_end:g} bohr, 15x15x12 angular grid; step = contraction + FP64 LDMD sweep + matching",
             "n_channel": N,
             "n_energy": nE,
             "n_sector": nS,
             "precision": "fp64",
         },
         "roofline": {
-            "kernel": "k_gj_update (rank-32 Gauss-Jordan update, FP64 MFMA)",
-            "bound": "mfma",
-            "achieved": achieved,
-            "peak": FP64_MFMA_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": None if achieved is None else achieved / FP64_MFMA_PEAK_TFLOPS,
-            "traffic": None,
+            "kernel": "k_gj_update<64> (rank-64 Gauss-Jordan update, two block steps per launch, FP64 MFMA)",
+            # 8 flop per algorithmic byte, below the MI355X FP64 ridge point (78.6 TF / 8 TB/s = 9.8)
+            "bound": "hbm",
+            "achieved": achieved_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": None if achieved_gbs is None else achieved_gbs / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_us": avg_ms * 1e3,
-            "matrices_per_launch": batch_per_launch,
+            "matrices_per_launch": matrices,
             "timed_launches": int(cnt[0]),
+            "mfma_tflops": achieved_tf,
+            "mfma_frac_of_fp64_peak": None if achieved_tf is None else achieved_tf / FP64_MFMA_PEAK_TFLOPS,
         },
         "sweep_tflops_credited": 4.0 * N**3 * nE * nS / (elapsed / args.steps) / 1e12,
         "max_unitarity_defect": defect,

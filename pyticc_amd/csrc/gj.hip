@@ -22,6 +22,8 @@
 // Row swaps compose into perm (original row at each position); the inverse of
 // the matrix is Out with its columns gathered through pinv = perm^-1, which the
 // consumers fuse into their own reads.
+#include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -353,7 +355,10 @@ __device__ __forceinline__ double2 buffer_load_f64x2(__amdgpu_buffer_rsrc_t r, i
 
 // Rank-KB update (KB = NB for one step, 2 NB for a pair of steps):
 //   Out[i][:] = (src2(i) < 0 ? 0 : X[src2(i)][:]) + AT[:, i]^T . X[brow][:],
-// the columns [k0, k0+KB) replaced by AT^T.  AT holds the KB coefficient columns
+// the columns [k0, k0+KB) replaced by AT^T.  Only the 64-column tiles [T0, T1)
+// are produced; column j lands in Out[i * ldo + j - ocol0] (a full update has
+// ldo = Np, ocol0 = 0; the look-ahead of the next pair's panel columns writes
+// one tile into an Np x 2NB buffer).  AT holds the KB coefficient columns
 // transposed (KB rows of length Np, matrix stride 2 NB Np) so an MFMA A fragment
 // is 16 consecutive doubles.  One workgroup (4 waves, 32 x 32 each) walks a
 // 64-row band across a group of 64-column tiles: the band's A fragments stay in
@@ -362,14 +367,14 @@ __device__ __forceinline__ double2 buffer_load_f64x2(__amdgpu_buffer_rsrc_t r, i
 // and stores run.
 template <int KB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_update(
-    const double* __restrict__ X, double* __restrict__ Out, int Np, int k0, const double* __restrict__ AT,
-    const int* __restrict__ src2, const int* __restrict__ brow, int ngroups) {
+    const double* __restrict__ X, double* __restrict__ Out, long long ostride, int ldo, int ocol0, int Np, int k0,
+    const double* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1,
+    int ngroups) {
   constexpr int KK = KB / 4;
   __shared__ double sB[2][KB][64];
-  const int tiles = (Np + 63) / 64;
   const int b = blockIdx.y;
   const int band = blockIdx.x / ngroups, grp = blockIdx.x % ngroups;
-  const int jt0 = (grp * tiles) / ngroups, jt1 = ((grp + 1) * tiles) / ngroups;
+  const int jt0 = T0 + (grp * (T1 - T0)) / ngroups, jt1 = T0 + ((grp + 1) * (T1 - T0)) / ngroups;
   if (jt0 >= jt1) return;
   const int t = threadIdx.x;
   const int wave = t >> 6, lane = t & 63, lr = lane & 15, lq = lane >> 4;
@@ -377,7 +382,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const int jw = (wave & 1) * 32;
   const bool rows_ok = i0 < Np;
   const double* Xb = X + (size_t)b * Np * Np;
-  double* Ob = Out + (size_t)b * Np * Np;
+  double* Ob = Out + (size_t)b * ostride - ocol0;
   const auto rX = make_rsrc(Xb, (unsigned)(sizeof(double) * Np * Np));
   const auto rA = make_rsrc(AT + (size_t)b * 2 * NB * Np, (unsigned)(sizeof(double) * KB * Np));
   const auto rS = make_rsrc(src2 + (size_t)b * Np, (unsigned)(sizeof(int) * Np));
@@ -459,7 +464,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          double* orow = Ob + (size_t)(i0 + 16 * bi + mfma_drow(lq, r)) * Np + j0 + lr;
+          double* orow = Ob + (size_t)(i0 + 16 * bi + mfma_drow(lq, r)) * ldo + j0 + lr;
 #pragma unroll
           for (int bj = 0; bj < 2; ++bj) orow[16 * bj] = acc[bi][bj][r];
         }
@@ -477,7 +482,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          double* orow = Ob + (size_t)(i0 + 16 * bi + mfma_drow(lq, r)) * Np + j0 + lr;
+          double* orow = Ob + (size_t)(i0 + 16 * bi + mfma_drow(lq, r)) * ldo + j0 + lr;
 #pragma unroll
           for (int bj = 0; bj < 2; ++bj) orow[16 * bj] = pv[bi][r][bj];
         }
@@ -522,8 +527,8 @@ __global__ void k_unpermute(const double* __restrict__ in, double* __restrict__ 
   }
 }
 
-// Live timing of the rank-NB update kernel with HIP events: every `every`-th
-// inversion has each of its update launches bracketed by an event pair.
+// Live timing of the rank-2NB update kernel with HIP events: every `every`-th
+// inversion has each of its rank-2NB update launches bracketed by an event pair.
 struct UpdateProfile {
   std::mutex mu;
   bool on = false;
@@ -599,24 +604,94 @@ int launch_panel(const double* X, long long bstride, int ld, int col0, int batch
 }
 
 template <int KB>
-int launch_update(const double* cur, double* nxt, int batch, int Np, int k0, const double* AT, const int* src2,
-                  const int* brow, bool sample, hipStream_t stream) {
-  const int tiles = (Np + 63) / 64;
-  // enough workgroups to fill the chip twice over; each walks tiles/ngroups tiles
-  int ngroups = (2048 + tiles * batch - 1) / (tiles * batch);
-  if (ngroups > tiles) ngroups = tiles;
+int launch_update(const double* cur, double* out, long long ostride, int ldo, int ocol0, int batch, int Np, int k0,
+                  const double* AT, const int* src2, const int* brow, int T0, int T1, bool sample,
+                  hipStream_t stream) {
+  const int bands = (Np + 63) / 64;
+  const int span = T1 - T0;
+  // enough workgroups to fill the chip twice over; each walks span/ngroups tiles
+  int ngroups = (2048 + bands * batch - 1) / (bands * batch);
+  if (ngroups > span) ngroups = span;
   if (ngroups < 1) ngroups = 1;
+  // live timing covers the rank-2NB launches (the sweep's dominant kernel)
+  sample = sample && KB == 2 * NB;
   size_t slot = 0;
   if (sample) {
     slot = g_prof.used++;
     g_prof.batch[slot] = batch * (KB / NB);  // block steps performed by this launch
     TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
   }
-  hipLaunchKernelGGL((k_gj_update<KB>), dim3(tiles * ngroups, batch), dim3(256), 0, stream, cur, nxt, Np, k0, AT,
-                     src2, brow, ngroups);
+  hipLaunchKernelGGL((k_gj_update<KB>), dim3(bands * ngroups, batch), dim3(256), 0, stream, cur, out, ostride, ldo,
+                     ocol0, Np, k0, AT, src2, brow, T0, T1, ngroups);
   TICC_CHECK_LAUNCH();
   if (sample) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
   return TICC_OK;
+}
+
+struct Carve {  // scratch carve-up of gj_invert_raw
+  double *P1, *P2, *Z, *AT;
+  int *s1, *s2, *src2, *p1, *p2, *brow;
+};
+
+Carve carve(double* pout, int* src, int* piv, int batch, int Np) {
+  Carve c;
+  const size_t ps = (size_t)batch * Np * NB;
+  c.P1 = pout;
+  c.P2 = pout + ps;
+  c.Z = pout + 2 * ps;
+  c.AT = pout + 3 * ps;  // 2NB x Np per matrix
+  const size_t ms = (size_t)batch * Np;
+  c.s1 = src;
+  c.s2 = src + ms;
+  c.src2 = src + 2 * ms;
+  const size_t ls = (size_t)batch * NB;
+  c.p1 = piv;
+  c.p2 = piv + ls;
+  c.brow = piv + 2 * ls;  // 2NB per matrix
+  return c;
+}
+
+// Panels and coefficients of the pair of steps (k0, k0+NB), read from the
+// panel columns held in Xc (matrix stride xs, leading dimension ldx, the
+// pair's first column at xcol); AT, src2 and brow receive the update operands.
+int pair_chain(const double* Xc, long long xs, int ldx, int xcol, int batch, int Np, int k0, const Carve& c,
+               double* AT, int* src2, int* brow, int* perm, int* pinv, hipStream_t st) {
+  const int k1 = k0 + NB;
+  const int rtiles = (Np + 63) / 64;
+  const long long pn = (long long)Np * NB;
+  const long long at_stride = 2LL * NB * Np;
+  TICC_RETURN_IF(launch_panel(Xc, xs, ldx, xcol, batch, Np, k0, c.P1, c.s1, c.p1, perm, pinv, PanelMaps{}, st));
+  // columns of step k+1 after step k: Z = X_{k+1}[:, k1:k1+NB]
+  hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, st, c.P1, Xc, xs, ldx, xcol + NB, c.s1, Xc, xs,
+                     ldx, xcol + NB, c.p1, k0, Np, c.Z, 0LL);
+  TICC_CHECK_LAUNCH();
+  PanelMaps m;
+  m.PoutT = AT + (size_t)NB * Np;
+  m.tstride = at_stride;
+  m.prev_src = c.s1;
+  m.prev_piv = c.p1;
+  m.prev_k0 = k0;
+  m.src2 = src2;
+  m.brow = brow;
+  TICC_RETURN_IF(launch_panel(c.Z, pn, NB, 0, batch, Np, k1, c.P2, c.s2, c.p2, perm, pinv, m, st));
+  // combined step-k coefficients A1 = mask P1[s2] + P2 . P1[p2], stored transposed in AT[0:NB]
+  hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, st, c.P2, c.P1, pn, NB, 0, c.s2, c.P1, pn, NB,
+                     0, c.p2, k1, Np, AT, at_stride);
+  TICC_CHECK_LAUNCH();
+  return TICC_OK;
+}
+
+int single_step(double* cur, double* nxt, int batch, int Np, int k0, const Carve& c, int* perm, int* pinv,
+                bool sample, hipStream_t st) {
+  const long long n2 = (long long)Np * Np;
+  PanelMaps m;
+  m.PoutT = c.AT;
+  m.tstride = 2LL * NB * Np;
+  m.src2 = c.src2;
+  m.brow = c.brow;
+  TICC_RETURN_IF(launch_panel(cur, n2, Np, k0, batch, Np, k0, c.P1, c.s1, c.p1, perm, pinv, m, st));
+  return launch_update<NB>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT, c.src2, c.brow, 0, (Np + 63) / 64,
+                           sample, st);
 }
 }  // namespace
 
@@ -624,24 +699,10 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
                   double* pout, int* src, int* piv, hipStream_t stream) {
   if (Np % NB != 0 || Np > MAX_NP || Np <= 0 || batch <= 0)
     return fail(TICC_ERR_VALUE, "gj_invert_raw: invalid Np=%d batch=%d", Np, batch);
-  // scratch carve-up: GJ_POUT_PANELS, GJ_SRC_MAPS and GJ_PIV_LISTS per matrix
-  const size_t ps = (size_t)batch * Np * NB;
-  double* P1 = pout;
-  double* P2 = pout + ps;
-  double* Z = pout + 2 * ps;
-  double* AT = pout + 3 * ps;  // 2 NB x Np per matrix
-  const long long at_stride = 2LL * NB * Np;
-  int* s1 = src;
-  int* s2 = src + (size_t)batch * Np;
-  int* src2 = src + 2 * (size_t)batch * Np;
-  int* p1 = piv;
-  int* p2 = piv + batch * NB;
-  int* brow = piv + 2 * batch * NB;  // 2 NB per matrix
+  const Carve c = carve(pout, src, piv, batch, Np);
   const long long n2 = (long long)Np * Np;
-  const long long pn = (long long)Np * NB;
-  double* cur = bufA;
-  double* nxt = bufB;
-  const int rtiles = (Np + 63) / 64;
+  const int tiles = (Np + 63) / 64;
+  double* buf[2] = {bufA, bufB};
   bool sample = false;
   if (g_prof.on) {
     std::lock_guard<std::mutex> lock(g_prof.mu);
@@ -649,43 +710,18 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
              (g_prof.used + (size_t)(Np / NB) <= g_prof.events.size() / 2);
   }
   const int steps = Np / NB;
-  for (int k = 0; k < steps;) {
-    const int k0 = k * NB;
-    if (k + 1 == steps) {  // trailing single step: rank-NB update
-      PanelMaps m;
-      m.PoutT = AT;
-      m.tstride = at_stride;
-      m.src2 = src2;
-      m.brow = brow;
-      TICC_RETURN_IF(launch_panel(cur, n2, Np, k0, batch, Np, k0, P1, s1, p1, perm, pinv, m, stream));
-      TICC_RETURN_IF(launch_update<NB>(cur, nxt, batch, Np, k0, AT, src2, brow, sample, stream));
-      k += 1;
-    } else {
-      const int k1 = k0 + NB;
-      TICC_RETURN_IF(launch_panel(cur, n2, Np, k0, batch, Np, k0, P1, s1, p1, perm, pinv, PanelMaps{}, stream));
-      // look-ahead columns of step k+1: Z = X_{k+1}[:, k1:k1+NB]
-      hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, stream, P1, cur, n2, Np, k1, s1, cur, n2, Np,
-                         k1, p1, k0, Np, Z, 0LL);
-      TICC_CHECK_LAUNCH();
-      PanelMaps m;
-      m.PoutT = AT + (size_t)NB * Np;
-      m.tstride = at_stride;
-      m.prev_src = s1;
-      m.prev_piv = p1;
-      m.prev_k0 = k0;
-      m.src2 = src2;
-      m.brow = brow;
-      TICC_RETURN_IF(launch_panel(Z, pn, NB, 0, batch, Np, k1, P2, s2, p2, perm, pinv, m, stream));
-      // combined step-k coefficients A1 = mask P1[s2] + P2 . P1[p2], stored transposed in AT[0:NB]
-      hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, stream, P2, P1, pn, NB, 0, s2, P1, pn, NB, 0,
-                         p2, k1, Np, AT, at_stride);
-      TICC_CHECK_LAUNCH();
-      TICC_RETURN_IF(launch_update<2 * NB>(cur, nxt, batch, Np, k0, AT, src2, brow, sample, stream));
-      k += 2;
-    }
-    double* tmp = cur;
-    cur = nxt;
-    nxt = tmp;
+  const int pairs = steps / 2;
+  for (int q = 0; q < pairs; ++q) {
+    const int k0 = 2 * q * NB;
+    double* cur = buf[q & 1];
+    TICC_RETURN_IF(pair_chain(cur, n2, Np, k0, batch, Np, k0, c, c.AT, c.src2, c.brow, perm, pinv, stream));
+    TICC_RETURN_IF(launch_update<2 * NB>(cur, buf[(q + 1) & 1], n2, Np, 0, batch, Np, k0, c.AT, c.src2,
+                                         c.brow, 0, tiles, sample, stream));
+  }
+  double* cur = buf[pairs & 1];
+  if (steps & 1) {  // trailing single step: rank-NB update
+    TICC_RETURN_IF(single_step(cur, buf[(pairs + 1) & 1], batch, Np, (steps - 1) * NB, c, perm, pinv, sample, stream));
+    cur = buf[(pairs + 1) & 1];
   }
   *result = cur;
   return TICC_OK;

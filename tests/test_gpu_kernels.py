@@ -30,7 +30,7 @@ def test_dgemm_batched_matches_numpy(gpu_device, m, n, k, batch):
     np.testing.assert_allclose(_device.to_host(dC), expect, rtol=1e-12, atol=1e-12 * np.sqrt(k))
 
 
-@pytest.mark.parametrize("N,batch", [(1, 1), (2, 3), (31, 2), (33, 4), (66, 5), (100, 3), (800, 2)])
+@pytest.mark.parametrize("N,batch", [(1, 1), (2, 3), (31, 2), (33, 4), (66, 5), (100, 3), (130, 2), (200, 3), (800, 2)])
 def test_invert_batched_matches_numpy(gpu_device, N, batch):
     rng = np.random.default_rng(N)
     A = rng.standard_normal((batch, N, N)) + np.eye(N) * 0.1
