@@ -90,19 +90,26 @@ __device__ __forceinline__ void wave_argmax_dpp(double& v, int& p) {
   dpp_argmax_step<0x124>(v, p);  // row_ror:4
   dpp_argmax_step<0x122>(v, p);  // row_ror:2
   dpp_argmax_step<0x121>(v, p);  // row_ror:1
-  double bv = readlane_f64(v, 0);
-  int bp = __builtin_amdgcn_readlane(p, 0);
+  double rv[4];
+  int rp[4];
 #pragma unroll
-  for (int r = 1; r < 4; ++r) {
-    const double rv = readlane_f64(v, 16 * r);
-    const int rp = __builtin_amdgcn_readlane(p, 16 * r);
-    if (better(rv, rp, bv, bp)) {
-      bv = rv;
-      bp = rp;
+  for (int r = 0; r < 4; ++r) {
+    rv[r] = readlane_f64(v, 16 * r);
+    rp[r] = __builtin_amdgcn_readlane(p, 16 * r);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r += 2) {
+    if (better(rv[r + 1], rp[r + 1], rv[r], rp[r])) {
+      rv[r] = rv[r + 1];
+      rp[r] = rp[r + 1];
     }
   }
-  v = bv;
-  p = bp;
+  if (better(rv[2], rp[2], rv[0], rp[0])) {
+    rv[0] = rv[2];
+    rp[0] = rp[2];
+  }
+  v = rv[0];
+  p = rp[0];
 }
 
 // Panel: one workgroup per matrix; thread t owns rows t + THREADS q
@@ -114,20 +121,19 @@ __device__ __forceinline__ void wave_argmax_dpp(double& v, int& p) {
 // the row swap to its own position records and performs the in-place
 // Gauss-Jordan update, streaming the pivot row from LDS.
 template <int THREADS, int RPT>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 256, THREADS / 256))) void k_gj_panel(const double* __restrict__ X, long long bstride, int ld, int col0, int Np, int k0,
-                                                      double* __restrict__ Pout, int* __restrict__ src,
-                                                      int* __restrict__ piv, int* __restrict__ perm,
-                                                      int* __restrict__ pinv, double* __restrict__ PoutT,
-                                                      long long tstride, const int* __restrict__ prev_src,
-                                                      const int* __restrict__ prev_piv, int prev_k0,
-                                                      int* __restrict__ src2, int* __restrict__ brow) {
+__device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long long bstride, int ld, int col0,
+                                              int Np, int k0, double* __restrict__ Pout, int* __restrict__ src,
+                                              int* __restrict__ piv, int* __restrict__ perm, int* __restrict__ pinv,
+                                              double* __restrict__ PoutT, long long tstride,
+                                              const int* __restrict__ prev_src, const int* __restrict__ prev_piv,
+                                              int prev_k0, int* __restrict__ src2, int* __restrict__ brow, int b) {
   constexpr int NW = THREADS / 64;
-  __shared__ double s_vals[2][NW][NB];
+  __shared__ double s_vals[2][NW][RPT][NB];
+  __shared__ double s_inv[NB];
   __shared__ double s_v[2][NW];
   __shared__ int s_pos[2][NW];
   __shared__ int s_row[2][NW];
 
-  const int b = blockIdx.x;
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = t >> 6;
@@ -178,14 +184,15 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
     double wv = cv;
     int wp = cp;
     wave_argmax_dpp(wv, wp);
-    if (cq >= 0 && cp == wp) {  // this lane holds the wave's winning row
+    // the lane holding the wave's winning row publishes it; each of a thread's
+    // rows has its own LDS slot so the write needs no register select
 #pragma unroll
-      for (int q = 0; q < RPT; ++q)
-        if (q == cq) {
+    for (int q = 0; q < RPT; ++q) {
+      if (cq == q && cp == wp) {
 #pragma unroll
-          for (int l = 0; l < NB; ++l) s_vals[buf][wave][l] = v[q][l];
-          s_row[buf][wave] = t + THREADS * q;
-        }
+        for (int l = 0; l < NB; ++l) s_vals[buf][wave][q][l] = v[q][l];
+        s_row[buf][wave] = t + THREADS * q;
+      }
     }
     if (lane == 0) {
       s_v[buf][wave] = wv;
@@ -204,15 +211,22 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
       }
     }
     const int pr = s_row[buf][bw];
-    const double* prow = s_vals[buf][bw];
+    const double* prow = s_vals[buf][bw][pr / THREADS];
     const double inv = 1.0 / prow[j];
-    // non-pivot rows: a_rl -= (a_rj / p) p_l, a_rj = -a_rj / p; the pivot row is
-    // rescaled afterwards in a branch taken by a single lane.
+    // Gauss-Jordan step with the pivot row's scaling deferred: every other row
+    // r takes a_rl -= (a_rj / p) p_l, a_rj = -a_rj / p; the pivot row keeps its
+    // values with a_rj = 1.  A row that was a pivot row stays p times its true
+    // values through the later columns of the panel (the update is linear in
+    // the row) and is multiplied by 1/p (kept in s_inv) once at the end.  The
+    // pivot row's own update is a no-op through g = 0, so no lane branches.
+    if (t == 0) s_inv[j] = inv;
     double g[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
+      const bool is_piv = (t + THREADS * q == pr);
       if (pos[q] == c) pos[q] = bp;
-      g[q] = v[q][j] * inv;
+      pos[q] = is_piv ? c : pos[q];
+      g[q] = is_piv ? 0.0 : v[q][j] * inv;
     }
 #pragma unroll
     for (int l = 0; l < NB; ++l) {
@@ -221,14 +235,17 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
       for (int q = 0; q < RPT; ++q) v[q][l] = fma(-g[q], pl, v[q][l]);
     }
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      v[q][j] = -g[q];
-      if (t + THREADS * q == pr) {
-        pos[q] = c;
+    for (int q = 0; q < RPT; ++q) v[q][j] = (t + THREADS * q == pr) ? 1.0 : -g[q];
+  }
+
+  // Deferred pivot-row scaling of this panel's pivot rows.
+  __syncthreads();
 #pragma unroll
-        for (int l = 0; l < NB; ++l) v[q][l] = prow[l] * inv;
-        v[q][j] = inv;
-      }
+  for (int q = 0; q < RPT; ++q) {
+    if (pos[q] >= k0 && pos[q] < k0 + NB) {
+      const double ri = s_inv[pos[q] - k0];
+#pragma unroll
+      for (int l = 0; l < NB; ++l) v[q][l] *= ri;
     }
   }
 
@@ -270,6 +287,16 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
     perm[(size_t)b * Np + pos[q]] = pold[q];
     if (k0 + NB >= Np) pinv[(size_t)b * Np + pold[q]] = pos[q];
   }
+}
+
+template <int THREADS, int RPT>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 256, THREADS / 256))) void k_gj_panel(
+    const double* __restrict__ X, long long bstride, int ld, int col0, int Np, int k0, double* __restrict__ Pout,
+    int* __restrict__ src, int* __restrict__ piv, int* __restrict__ perm, int* __restrict__ pinv,
+    double* __restrict__ PoutT, long long tstride, const int* __restrict__ prev_src, const int* __restrict__ prev_piv,
+    int prev_k0, int* __restrict__ src2, int* __restrict__ brow) {
+  gj_panel_body<THREADS, RPT>(X, bstride, ld, col0, Np, k0, Pout, src, piv, perm, pinv, PoutT, tstride, prev_src,
+                              prev_piv, prev_k0, src2, brow, blockIdx.x);
 }
 
 // Small panel GEMM (FP64 MFMA) of the paired block step:
@@ -366,17 +393,12 @@ __device__ __forceinline__ double2 buffer_load_f64x2(__amdgpu_buffer_rsrc_t r, i
 // rows (into the idle LDS buffer) are loaded while the current tile's MFMAs
 // and stores run.
 template <int KB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_update(
-    const double* __restrict__ X, double* __restrict__ Out, long long ostride, int ldo, int ocol0, int Np, int k0,
-    const double* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1,
-    int ngroups) {
+__device__ __forceinline__ void gj_update_body(const double* __restrict__ X, double* __restrict__ Out,
+                                               long long ostride, int ldo, int ocol0, int Np, int k0,
+                                               const double* __restrict__ AT, const int* __restrict__ src2,
+                                               const int* __restrict__ brow, int b, int band, int jt0, int jt1,
+                                               int t, double (*sB)[KB][64]) {
   constexpr int KK = KB / 4;
-  __shared__ double sB[2][KB][64];
-  const int b = blockIdx.y;
-  const int band = blockIdx.x / ngroups, grp = blockIdx.x % ngroups;
-  const int jt0 = T0 + (grp * (T1 - T0)) / ngroups, jt1 = T0 + ((grp + 1) * (T1 - T0)) / ngroups;
-  if (jt0 >= jt1) return;
-  const int t = threadIdx.x;
   const int wave = t >> 6, lane = t & 63, lr = lane & 15, lq = lane >> 4;
   const int i0 = band * 64 + (wave >> 1) * 32;
   const int jw = (wave & 1) * 32;
@@ -516,6 +538,57 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   }
 }
 
+// Column tiles [jt0, jt1) of group `grp` out of `ngroups` over [T0, T1).
+__device__ __forceinline__ void group_tiles(int T0, int T1, int grp, int ngroups, int& jt0, int& jt1) {
+  jt0 = T0 + (grp * (T1 - T0)) / ngroups;
+  jt1 = T0 + ((grp + 1) * (T1 - T0)) / ngroups;
+}
+
+template <int KB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_update(
+    const double* __restrict__ X, double* __restrict__ Out, long long ostride, int ldo, int ocol0, int Np, int k0,
+    const double* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1,
+    int ngroups) {
+  __shared__ double sB[2][KB][64];
+  int jt0, jt1;
+  group_tiles(T0, T1, blockIdx.x % ngroups, ngroups, jt0, jt1);
+  if (jt0 >= jt1) return;
+  gj_update_body<KB>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, blockIdx.y, blockIdx.x / ngroups, jt0, jt1,
+                     threadIdx.x, sB);
+}
+
+// One launch with two jobs on disjoint matrices' data: the first `npanel`
+// workgroups factor the next pair's panel (one matrix each, as k_gj_panel), the
+// rest run column tiles [T0, T1) of the current pair's rank-2NB update as two
+// independent 256-thread halves (two 64-row bands).  Workgroups dispatch in
+// index order, so the latency-bound panels take their CUs first and the update
+// fills the others -- the panel chain of pair q+1 hides behind update q.
+template <int RPT>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_fused(
+    const double* __restrict__ PX, long long pbstride, int pld, int pcol0, int pk0, double* __restrict__ Pout,
+    int* __restrict__ psrc, int* __restrict__ ppiv, int* __restrict__ perm, int* __restrict__ pinv,
+    double* __restrict__ PoutT, long long tstride, const int* __restrict__ prev_src,
+    const int* __restrict__ prev_piv, int prev_k0, int* __restrict__ psrc2, int* __restrict__ pbrow, int npanel,
+    const double* __restrict__ X, double* __restrict__ Out, int Np, int k0, const double* __restrict__ AT,
+    const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1, int ngroups) {
+  if ((int)blockIdx.x < npanel) {
+    gj_panel_body<512, RPT>(PX, pbstride, pld, pcol0, Np, pk0, Pout, psrc, ppiv, perm, pinv, PoutT, tstride,
+                            prev_src, prev_piv, prev_k0, psrc2, pbrow, blockIdx.x);
+    return;
+  }
+  __shared__ double sB[2][2][2 * NB][64];
+  const int u = blockIdx.x - npanel;
+  const int band_pairs = ((Np + 63) / 64 + 1) / 2;
+  const int per_b = band_pairs * ngroups;
+  const int b = u / per_b, rem = u % per_b;
+  const int half = threadIdx.x >> 8;  // a half whose band lies past Np only joins the barriers
+  int jt0, jt1;
+  group_tiles(T0, T1, rem % ngroups, ngroups, jt0, jt1);
+  if (jt0 >= jt1) return;
+  gj_update_body<2 * NB>(X, Out, (long long)Np * Np, Np, 0, Np, k0, AT, src2, brow, b, 2 * (rem / ngroups) + half,
+                         jt0, jt1, threadIdx.x & 255, sB[half]);
+}
+
 __global__ void k_unpermute(const double* __restrict__ in, double* __restrict__ out, int Np,
                             const int* __restrict__ pinv) {
   const int b = blockIdx.y;
@@ -629,8 +702,8 @@ int launch_update(const double* cur, double* out, long long ostride, int ldo, in
 }
 
 struct Carve {  // scratch carve-up of gj_invert_raw
-  double *P1, *P2, *Z, *AT;
-  int *s1, *s2, *src2, *p1, *p2, *brow;
+  double *P1, *P2, *Z, *L, *AT[2];
+  int *s1, *s2, *src2[2], *p1, *p2, *brow[2];
 };
 
 Carve carve(double* pout, int* src, int* piv, int batch, int Np) {
@@ -639,15 +712,19 @@ Carve carve(double* pout, int* src, int* piv, int batch, int Np) {
   c.P1 = pout;
   c.P2 = pout + ps;
   c.Z = pout + 2 * ps;
-  c.AT = pout + 3 * ps;  // 2NB x Np per matrix
+  c.L = pout + 3 * ps;      // Np x 2NB per matrix: look-ahead columns of the next pair
+  c.AT[0] = pout + 5 * ps;  // 2NB x Np per matrix, one set per pair parity
+  c.AT[1] = pout + 7 * ps;
   const size_t ms = (size_t)batch * Np;
   c.s1 = src;
   c.s2 = src + ms;
-  c.src2 = src + 2 * ms;
+  c.src2[0] = src + 2 * ms;
+  c.src2[1] = src + 3 * ms;
   const size_t ls = (size_t)batch * NB;
   c.p1 = piv;
   c.p2 = piv + ls;
-  c.brow = piv + 2 * ls;  // 2NB per matrix
+  c.brow[0] = piv + 2 * ls;  // 2NB per matrix
+  c.brow[1] = piv + 4 * ls;
   return c;
 }
 
@@ -685,13 +762,36 @@ int single_step(double* cur, double* nxt, int batch, int Np, int k0, const Carve
                 bool sample, hipStream_t st) {
   const long long n2 = (long long)Np * Np;
   PanelMaps m;
-  m.PoutT = c.AT;
+  m.PoutT = c.AT[0];
   m.tstride = 2LL * NB * Np;
-  m.src2 = c.src2;
-  m.brow = c.brow;
+  m.src2 = c.src2[0];
+  m.brow = c.brow[0];
   TICC_RETURN_IF(launch_panel(cur, n2, Np, k0, batch, Np, k0, c.P1, c.s1, c.p1, perm, pinv, m, st));
-  return launch_update<NB>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT, c.src2, c.brow, 0, (Np + 63) / 64,
+  return launch_update<NB>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[0], c.src2[0], c.brow[0], 0, (Np + 63) / 64,
                            sample, st);
+}
+// Fused launch: the panel of step pk0 (reading columns [pcol0, pcol0+NB) of PX)
+// beside column tiles [T0, T1) of the rank-2NB update of the pair at k0.
+int launch_fused(const double* PX, long long pbstride, int pld, int pcol0, int pk0, double* pout, int* psrc,
+                 int* ppiv, int* perm, int* pinv, const PanelMaps& m, const double* X, double* Out, int batch, int Np,
+                 int k0, const double* AT, const int* src2, const int* brow, int T0, int T1, hipStream_t stream) {
+  const int band_pairs = ((Np + 63) / 64 + 1) / 2;
+  // about two workgroups per CU: longer tile walks amortise each band's
+  // prologue (A fragments, first C and B tiles) better than more, shorter ones
+  int ngroups = (512 + band_pairs * batch - 1) / (band_pairs * batch);
+  if (ngroups > T1 - T0) ngroups = T1 - T0;
+  if (ngroups < 1) ngroups = 1;
+  const int grid = batch + band_pairs * ngroups * batch;
+#define TICC_FUSED_ARGS                                                                                            \
+  PX, pbstride, pld, pcol0, pk0, pout, psrc, ppiv, perm, pinv, m.PoutT, m.tstride, m.prev_src, m.prev_piv,         \
+      m.prev_k0, m.src2, m.brow, batch, X, Out, Np, k0, AT, src2, brow, T0, T1, ngroups
+  if (Np <= 512)
+    hipLaunchKernelGGL((k_gj_fused<1>), dim3(grid), dim3(512), 0, stream, TICC_FUSED_ARGS);
+  else
+    hipLaunchKernelGGL((k_gj_fused<2>), dim3(grid), dim3(512), 0, stream, TICC_FUSED_ARGS);
+#undef TICC_FUSED_ARGS
+  TICC_CHECK_LAUNCH();
+  return TICC_OK;
 }
 }  // namespace
 
@@ -711,12 +811,45 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
   }
   const int steps = Np / NB;
   const int pairs = steps / 2;
+  const long long pn = (long long)Np * NB;
+  const long long ln = 2LL * NB * Np;  // look-ahead buffer: Np x 2NB per matrix
+  const int rtiles = (Np + 63) / 64;
+  if (pairs > 0) TICC_RETURN_IF(pair_chain(buf[0], n2, Np, 0, batch, Np, 0, c, c.AT[0], c.src2[0], c.brow[0], perm,
+                                           pinv, stream));
   for (int q = 0; q < pairs; ++q) {
     const int k0 = 2 * q * NB;
+    const int par = q & 1;
     double* cur = buf[q & 1];
-    TICC_RETURN_IF(pair_chain(cur, n2, Np, k0, batch, Np, k0, c, c.AT, c.src2, c.brow, perm, pinv, stream));
-    TICC_RETURN_IF(launch_update<2 * NB>(cur, buf[(q + 1) & 1], n2, Np, 0, batch, Np, k0, c.AT, c.src2,
-                                         c.brow, 0, tiles, sample, stream));
+    double* nxt = buf[(q + 1) & 1];
+    if (q + 1 == pairs) {
+      TICC_RETURN_IF(launch_update<2 * NB>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[par], c.src2[par], c.brow[par],
+                                           0, tiles, sample, stream));
+      break;
+    }
+    // Pair q+1 (columns kn, kn + NB) is factored while update q runs: its
+    // columns after pair q come from a one-tile look-ahead of update q.
+    const int kn = k0 + 2 * NB;
+    TICC_RETURN_IF(launch_update<2 * NB>(cur, c.L, ln, 2 * NB, kn, batch, Np, k0, c.AT[par], c.src2[par],
+                                         c.brow[par], kn / 64, kn / 64 + 1, false, stream));
+    const int th = tiles / 2;
+    TICC_RETURN_IF(launch_fused(c.L, ln, 2 * NB, 0, kn, c.P1, c.s1, c.p1, perm, pinv, PanelMaps{}, cur, nxt, batch,
+                                Np, k0, c.AT[par], c.src2[par], c.brow[par], 0, th, stream));
+    hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, stream, c.P1, c.L, ln, 2 * NB, NB, c.s1, c.L,
+                       ln, 2 * NB, NB, c.p1, kn, Np, c.Z, 0LL);
+    TICC_CHECK_LAUNCH();
+    PanelMaps m;
+    m.PoutT = c.AT[par ^ 1] + (size_t)NB * Np;
+    m.tstride = 2LL * NB * Np;
+    m.prev_src = c.s1;
+    m.prev_piv = c.p1;
+    m.prev_k0 = kn;
+    m.src2 = c.src2[par ^ 1];
+    m.brow = c.brow[par ^ 1];
+    TICC_RETURN_IF(launch_fused(c.Z, pn, NB, 0, kn + NB, c.P2, c.s2, c.p2, perm, pinv, m, cur, nxt, batch, Np, k0,
+                                c.AT[par], c.src2[par], c.brow[par], th, tiles, stream));
+    hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, stream, c.P2, c.P1, pn, NB, 0, c.s2, c.P1, pn,
+                       NB, 0, c.p2, kn + NB, Np, c.AT[par ^ 1], 2LL * NB * Np);
+    TICC_CHECK_LAUNCH();
   }
   double* cur = buf[pairs & 1];
   if (steps & 1) {  // trailing single step: rank-NB update

@@ -7,6 +7,10 @@
 //   Yeff  = Y_oo - Y_oc Z Y_co
 //   K     = -(Yeff diag N - diag N')^-1 (Yeff diag J - diag J')
 //   S     = (I + K^2)^-1 (I - K^2) - 2i (I + K^2)^-1 K     (== (I+iK)^-1 (I-iK))
+//   then two FP64 refinement sweeps against the residual of (I + iK) S = I - iK:
+//     E = (I - iK) - (I + iK) S,   S += (I + K^2)^-1 (I - iK) E
+//   (I + K^2 squares the conditioning of I + iK near a pole of K; the residual
+//   form restores the accuracy of a direct complex solve).
 // Ragged open/closed counts are handled by identity (inverted operands) or
 // zero (multiplied operands) padding inside uniform Np x Np batches.
 #include <cmath>
@@ -154,6 +158,29 @@ __global__ void k_build_mp(const double* __restrict__ KK, const int* __restrict_
   }
 }
 
+// Residual of (I + iK) S = I - iK on the open block, from KSi = K Im S and
+// KSr = K Re S (both overwritten): Re E = I - Re S + K Im S,
+// Im E = -K - Im S - K Re S.  Zero outside the open block.
+__global__ void k_smat_residual(const double* __restrict__ Sr, const double* __restrict__ Si,
+                                const double* __restrict__ K, const int* __restrict__ ncl, int N, int Np,
+                                double* __restrict__ KSi_Er, double* __restrict__ KSr_Ei) {
+  const int e = blockIdx.y;
+  const int no = N - ncl[e];
+  const size_t n2 = (size_t)Np * Np;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n2; idx += (size_t)gridDim.x * blockDim.x) {
+    const int a = (int)(idx / Np);
+    const int b = (int)(idx % Np);
+    const size_t o = e * n2 + idx;
+    double er = 0.0, ei = 0.0;
+    if (a < no && b < no) {
+      er = ((a == b) ? 1.0 : 0.0) - Sr[o] + KSi_Er[o];
+      ei = -K[o] - Si[o] - KSr_Ei[o];
+    }
+    KSi_Er[o] = er;
+    KSr_Ei[o] = ei;
+  }
+}
+
 int grid1(long long n) {
   long long b = (n + 255) / 256;
   if (b > 2048) b = 2048;
@@ -199,8 +226,8 @@ int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, con
     off += (bytes + 255) / 256 * 256;
     return o;
   };
-  size_t ob[6];
-  for (int i = 0; i < 6; ++i) ob[i] = take(sizeof(double) * EC * n2);
+  size_t ob[7];
+  for (int i = 0; i < 7; ++i) ob[i] = take(sizeof(double) * EC * n2);
   const size_t oB = take(sizeof(double) * n2);
   const size_t oRef = take(sizeof(double) * 5 * (size_t)nE * N);
   const size_t oE = take(sizeof(double) * nE);
@@ -217,8 +244,8 @@ int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, con
   void* base = nullptr;
   TICC_RETURN_IF(workspace(WS_MATCH, off, &base));
   char* w = static_cast<char*>(base);
-  double* b[6];
-  for (int i = 0; i < 6; ++i) b[i] = reinterpret_cast<double*>(w + ob[i]);
+  double* b[7];
+  for (int i = 0; i < 7; ++i) b[i] = reinterpret_cast<double*>(w + ob[i]);
   double* Bp = reinterpret_cast<double*>(w + oB);
   double* ref = reinterpret_cast<double*>(w + oRef);
   double *Jv = ref, *Nv = ref + (size_t)nE * N, *Jd = ref + 2 * (size_t)nE * N, *Nd = ref + 3 * (size_t)nE * N,
@@ -295,6 +322,25 @@ int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, con
                                  stream));
     TICC_RETURN_IF(dgemm_batched(false, false, Np, Np, Np, -2.0, b[3], Np, n2, b[4], Np, n2, 0.0, b[5], Np, n2, ne,
                                  stream));
+    // 9. refinement: E -> (b1, b2); T = (I - iK) E -> (b1, b6); S += Mi T
+    for (int it = 0; it < 2; ++it) {
+      TICC_RETURN_IF(dgemm_batched(false, false, Np, Np, Np, 1.0, b[4], Np, n2, b[5], Np, n2, 0.0, b[1], Np, n2, ne,
+                                   stream));
+      TICC_RETURN_IF(dgemm_batched(false, false, Np, Np, Np, 1.0, b[4], Np, n2, b[0], Np, n2, 0.0, b[2], Np, n2, ne,
+                                   stream));
+      hipLaunchKernelGGL(k_smat_residual, dim3(ge, ne), dim3(256), 0, stream, b[0], b[5], b[4], ncc, N, Np, b[1],
+                         b[2]);
+      TICC_CHECK_LAUNCH();
+      TICC_CHECK_HIP(hipMemcpyAsync(b[6], b[2], sizeof(double) * ne * n2, hipMemcpyDeviceToDevice, stream));
+      TICC_RETURN_IF(dgemm_batched(false, false, Np, Np, Np, -1.0, b[4], Np, n2, b[1], Np, n2, 1.0, b[6], Np, n2, ne,
+                                   stream));
+      TICC_RETURN_IF(dgemm_batched(false, false, Np, Np, Np, 1.0, b[4], Np, n2, b[2], Np, n2, 1.0, b[1], Np, n2, ne,
+                                   stream));
+      TICC_RETURN_IF(dgemm_batched(false, false, Np, Np, Np, 1.0, b[3], Np, n2, b[1], Np, n2, 1.0, b[0], Np, n2, ne,
+                                   stream));
+      TICC_RETURN_IF(dgemm_batched(false, false, Np, Np, Np, 1.0, b[3], Np, n2, b[6], Np, n2, 1.0, b[5], Np, n2, ne,
+                                   stream));
+    }
     TICC_RETURN_IF(unpad(b[0], Np, S_re + (size_t)e0 * N * N, (long long)N * N, N, ne, stream));
     TICC_RETURN_IF(unpad(b[5], Np, S_im + (size_t)e0 * N * N, (long long)N * N, N, ne, stream));
   }

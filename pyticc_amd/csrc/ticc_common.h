@@ -27,12 +27,13 @@ int aux_events(int n, hipEvent_t* out);
 // every internal square matrix (Np = round_up(N, GJ_NB)).
 constexpr int GJ_NB = 32;
 constexpr int MAX_NP = 1024;
-// Scratch of gj_invert_raw per matrix: pout holds 5 Np x NB panels (two
-// transformed panels, the look-ahead columns and the 2 NB transposed update
-// coefficients), src 3 Np row maps, piv 4 NB-entry row lists.
-constexpr int GJ_POUT_PANELS = 5;
-constexpr int GJ_SRC_MAPS = 3;
-constexpr int GJ_PIV_LISTS = 4;
+// Scratch of gj_invert_raw per matrix: pout holds 9 Np x NB panels (two
+// transformed panels, the look-ahead columns of one and of two steps, and the
+// 2 NB transposed update coefficients of two pairs), src 4 Np row maps, piv 6
+// NB-entry row lists.
+constexpr int GJ_POUT_PANELS = 9;
+constexpr int GJ_SRC_MAPS = 4;
+constexpr int GJ_PIV_LISTS = 6;
 
 inline int round_up(int n, int q) { return (n + q - 1) / q * q; }
 

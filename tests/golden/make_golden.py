@@ -1,0 +1,120 @@
+"""Generate the committed golden fixtures (run on the CPU; minutes for the N=800 case).
+
+    python tests/golden/make_golden.py [bessel] [cg] [config1] [config2]
+
+Fixtures (NumPy .npz, no pickled objects):
+  bessel_mpmath.npz   Riccati-Bessel j, n, j', n' (j = sqrt(pi x/2) J_{l+1/2}(x)) and the
+                      K log-derivative K'_nu/K_nu at integer and non-integer orders,
+                      evaluated with mpmath at 40 significant digits -- an independent
+                      high-precision check of the scipy-based oracle and of the device code.
+  cg_sympy.npz        Clebsch-Gordan coefficients from sympy.physics.wigner (exact
+                      rationals under a square root, rounded once to FP64).
+  smat_config1.npz    oracle S-matrices of the atom + rigid-rotor system (N = 66, J = 10,
+                      p = +1, 1350 sectors 3-30 bohr) at 5 of its 20 energies.
+  smat_config2.npz    oracle S-matrices of the two-rotor system (N = 800, J = 4, p = -1,
+                      2000 sectors 4-40 bohr, 15 x 15 x 12 grid) at 3 of its 64 energies:
+                      the first 16 columns of S, its diagonal and the oracle's unitarity defect.
+
+Every fixture is written by the oracle (tests/../oracle) or by mpmath/sympy; nothing
+here reads the reference sources.
+"""
+
+from __future__ import annotations
+
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parents[1]))
+
+CONFIG1_ENERGIES = (0, 4, 9, 14, 19)
+CONFIG2_ENERGIES = (0, 31, 63)
+CONFIG2_COLUMNS = 16
+
+
+def make_bessel() -> None:
+    import mpmath as mp
+
+    mp.mp.dps = 40
+    ell = [0.0, 0.0, 1.0, 2.3, 3.0, 4.2, 10.0, 17.0, 20.3, 1.3, 0.7, 12.5, 30.0, 5.0, 0.5, 24.0]
+    x = [1.7, 0.05, 3.0, 4.2, 516.0, 9.0, 1.5, 60.0, 300.0, 2500.0, 1.99, 35.0, 45.0, 0.3, 1000.0, 31.0]
+    rows = []
+    for l, xv in zip(ell, x):
+        nu = mp.mpf(l) + mp.mpf(1) / 2
+        X = mp.mpf(xv)
+        s = mp.sqrt(mp.pi * X / 2)
+        J, Y = mp.besselj(nu, X), mp.bessely(nu, X)
+        Jp, Yp = mp.diff(lambda t: mp.besselj(nu, t), X), mp.diff(lambda t: mp.bessely(nu, t), X)
+        rows.append([float(s * J), float(s * Y), float(s * (Jp + J / (2 * X))), float(s * (Yp + Y / (2 * X)))])
+    nu_k = [0.5, 1.7, 1.5, 3.5, 10.5, 48.5, 0.5, 20.5, 2.7, 7.5]
+    x_k = [0.3, 3.1, 1.999, 2.0, 7.0, 2500.0, 100.0, 0.05, 476.0, 40.0]
+    klogd = []
+    for nu, xv in zip(nu_k, x_k):
+        f = lambda t: mp.besselk(nu, t)  # noqa: E731
+        klogd.append(float(mp.diff(f, xv) / f(xv)))
+    np.savez(HERE / "bessel_mpmath.npz", ell=np.array(ell), x=np.array(x), riccati=np.array(rows),
+             nu=np.array(nu_k), xk=np.array(x_k), k_logd=np.array(klogd))
+
+
+def make_cg() -> None:
+    from sympy.physics.wigner import clebsch_gordan
+
+    rng = np.random.default_rng(11)
+    cases = []
+    while len(cases) < 200:
+        j1, j2 = (int(v) for v in rng.integers(0, 9, 2))
+        J = int(rng.integers(abs(j1 - j2), j1 + j2 + 1))
+        m1 = int(rng.integers(-j1, j1 + 1))
+        m2 = int(rng.integers(-j2, j2 + 1))
+        if abs(m1 + m2) > J:
+            continue
+        val = float(clebsch_gordan(j1, j2, J, m1, m2, m1 + m2).evalf(30))
+        cases.append((j1, m1, j2, m2, J, val))
+    arr = np.array(cases)
+    np.savez(HERE / "cg_sympy.npz", idx=arr[:, :5].astype(np.int64), value=arr[:, 5])
+
+
+def _oracle_smat(workload, energy_index):
+    from oracle import matching as omatch
+    from oracle import pipeline as opipe
+    from tests.helpers import oracle_problem
+
+    _, _, prob = oracle_problem(workload)
+    E = workload.energies[list(energy_index)]
+    _, _, _, S = opipe.solve(prob, E)
+    return E, S, [omatch.unitarity_defect(s) if s.size else 0.0 for s in S]
+
+
+def make_config1() -> None:
+    from pyticc_amd import synthetic
+
+    w = synthetic.atom_rotor()
+    E, S, defect = _oracle_smat(w, CONFIG1_ENERGIES)
+    out = {"energy_index": np.array(CONFIG1_ENERGIES), "energies": E, "defect": np.array(defect)}
+    for i, s in enumerate(S):
+        out[f"S{i}"] = s
+    np.savez_compressed(HERE / "smat_config1.npz", **out)
+
+
+def make_config2() -> None:
+    from pyticc_amd import synthetic
+
+    w = synthetic.rotor_rotor()
+    E, S, defect = _oracle_smat(w, CONFIG2_ENERGIES)
+    out = {"energy_index": np.array(CONFIG2_ENERGIES), "energies": E, "defect": np.array(defect),
+           "n_open": np.array([s.shape[0] for s in S])}
+    for i, s in enumerate(S):
+        out[f"S{i}_cols"] = s[:, :CONFIG2_COLUMNS]
+        out[f"S{i}_diag"] = np.diag(s).copy()
+    np.savez_compressed(HERE / "smat_config2.npz", **out)
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:] or ["bessel", "cg", "config1", "config2"]
+    for name in which:
+        t = time.perf_counter()
+        {"bessel": make_bessel, "cg": make_cg, "config1": make_config1, "config2": make_config2}[name]()
+        print(f"{name}: {time.perf_counter() - t:.1f} s", flush=True)

@@ -1,0 +1,71 @@
+"""HIP path against the committed golden fixtures: mpmath Bessel tables and oracle
+S-matrices of the atom + rigid-rotor system (N = 66, all 1350 sectors) and of the
+two-rotor benchmark system at its full size (N = 800, 2000 sectors).
+
+Tolerance at full size: two equally valid FP64 restatements of the same algorithm
+differ there by up to 4.2e-10 in S (the oracle with explicit inverses instead of
+solves, tests/golden/noise_floor.py, measured at the fixture's energies), so the
+full-size comparison uses 1e-9; the reduced systems use 1e-10."""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import pyticc_amd as ticc
+from oracle import matching as omatch
+from pyticc_amd import synthetic
+from pyticc_amd.match.bessel import k_logd_batch, riccati_bessel_batch
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+TOL_S = 1.0e-10
+TOL_S_FULL = 1.0e-9  # FP64 noise floor of the N = 800, 2000-sector system: 4.2e-10
+
+
+def test_device_bessel_matches_mpmath(gpu_device):
+    d = np.load(GOLDEN / "bessel_mpmath.npz")
+    got = riccati_bessel_batch(d["ell"], d["x"], gpu_device.index)
+    for g, ref in zip(got, d["riccati"]):
+        env = max(np.hypot(ref[0], ref[1]), 1e-300)
+        envp = max(np.hypot(ref[2], ref[3]), 1e-300)
+        np.testing.assert_allclose(g[:2], ref[:2], rtol=0, atol=2e-13 * env)
+        np.testing.assert_allclose(g[2:], ref[2:], rtol=0, atol=2e-13 * envp)
+    np.testing.assert_allclose(k_logd_batch(d["nu"], d["xk"], gpu_device.index), d["k_logd"], rtol=1e-13)
+
+
+def test_solve_config1_matches_fixture(gpu_device):
+    d = np.load(GOLDEN / "smat_config1.npz")
+    w = synthetic.atom_rotor()
+    grid = ticc.prepare_potential(w.system, w.boundaries, w.half_steps, **w.quadrature)
+    res = ticc.solve(w.system, w.energies, grid, ticc.Propagation(device="gpu"))
+    worst = 0.0
+    for i, e in enumerate(d["energy_index"]):
+        ref = d[f"S{i}"]
+        assert res.Smat[e].shape == ref.shape
+        worst = max(worst, float(np.max(np.abs(res.Smat[e] - ref))))
+    assert worst <= TOL_S, worst
+    assert max(omatch.unitarity_defect(s) for s in res.Smat if s.size) < 1e-10
+
+
+def test_solve_config2_full_size_matches_fixture(gpu_device):
+    path = GOLDEN / "smat_config2.npz"
+    d = np.load(path)
+    w = synthetic.rotor_rotor()
+    idx = d["energy_index"]
+    grid = ticc.prepare_potential(w.system, w.boundaries, w.half_steps, **w.quadrature)
+    res = ticc.solve(w.system, w.energies[idx], grid, ticc.Propagation(device="gpu", memory_mb=4096.0))
+    worst = 0.0
+    for i in range(idx.size):
+        S = res.Smat[i]
+        assert S.shape[0] == int(d["n_open"][i])
+        if S.size == 0:  # no open channel at this energy
+            continue
+        cols = d[f"S{i}_cols"]
+        worst = max(worst, float(np.max(np.abs(S[:, : cols.shape[1]] - cols))))
+        worst = max(worst, float(np.max(np.abs(np.diag(S) - d[f"S{i}_diag"]))))
+        # the device S is as unitary as the oracle's on the same inputs
+        assert omatch.unitarity_defect(S) <= 10.0 * float(d["defect"][i]) + 1e-12
+    print(f"config-2 full size: max |dS| = {worst:.3e}, oracle defects {d['defect']}")
+    assert worst <= TOL_S_FULL, worst
