@@ -387,19 +387,23 @@ __device__ __forceinline__ double2 buffer_load_f64x2(__amdgpu_buffer_rsrc_t r, i
 // ldo = Np, ocol0 = 0; the look-ahead of the next pair's panel columns writes
 // one tile into an Np x 2NB buffer).  AT holds the KB coefficient columns
 // transposed (KB rows of length Np, matrix stride 2 NB Np) so an MFMA A fragment
-// is 16 consecutive doubles.  One workgroup (4 waves, 32 x 32 each) walks a
-// 64-row band across a group of 64-column tiles: the band's A fragments stay in
+// is 16 consecutive doubles.  A workgroup of NBANDS x 4 waves (32 x 32 each)
+// walks NBANDS adjacent 64-row bands across a group of 64-column tiles, sharing
+// each staged B tile between the bands: the bands' A fragments stay in
 // registers, and the next tile's C rows (into the idle accumulator set) and B
 // rows (into the idle LDS buffer) are loaded while the current tile's MFMAs
 // and stores run.
-template <int KB>
+template <int KB, int NBANDS>
 __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, double* __restrict__ Out,
                                                long long ostride, int ldo, int ocol0, int Np, int k0,
                                                const double* __restrict__ AT, const int* __restrict__ src2,
-                                               const int* __restrict__ brow, int b, int band, int jt0, int jt1,
+                                               const int* __restrict__ brow, int b, int band0, int jt0, int jt1,
                                                int t, double (*sB)[KB][64]) {
   constexpr int KK = KB / 4;
-  const int wave = t >> 6, lane = t & 63, lr = lane & 15, lq = lane >> 4;
+  constexpr int RP = 8 * NBANDS;  // B-tile rows staged per pass by the whole workgroup
+  const int th = t & 255;         // thread within its band's 4 waves
+  const int band = band0 + (NBANDS > 1 ? (t >> 8) : 0);
+  const int wave = th >> 6, lane = th & 63, lr = lane & 15, lq = lane >> 4;
   const int i0 = band * 64 + (wave >> 1) * 32;
   const int jw = (wave & 1) * 32;
   const bool rows_ok = i0 < Np;
@@ -425,10 +429,10 @@ __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, dou
           rS, rows_ok ? 4 * (i0 + 16 * bi + mfma_drow(lq, r)) : kOutOfRange, 0, 0);
       sr[bi][r] = rows_ok ? v : -1;
     }
-  const int half = t >> 5, c2 = (t & 31) * 2;
-  int bro[KB / 8];
+  const int bk = t >> 5, c2 = (t & 31) * 2;
+  int bro[KB / RP];
 #pragma unroll
-  for (int h = 0; h < KB / 8; ++h) bro[h] = brb[h * 8 + half];
+  for (int h = 0; h < KB / RP; ++h) bro[h] = brb[h * RP + bk];
 
   // 0: update tile, 1: coefficient (panel) columns, 2: outside the matrix
   auto kind = [&](int J) {
@@ -449,19 +453,19 @@ __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, dou
         for (int bj = 0; bj < 2; ++bj) c[bi][bj][r] = buffer_load_f64(rX, base + 128 * bj);
       }
   };
-  double2 bst[KB / 8];
+  double2 bst[KB / RP];
   auto load_b = [&](int J) {
     const bool in = (J * 64 + c2) < Np;
 #pragma unroll
-    for (int h = 0; h < KB / 8; ++h)
+    for (int h = 0; h < KB / RP; ++h)
       bst[h] = buffer_load_f64x2(rX, in ? 8 * (bro[h] * Np + J * 64 + c2) : kOutOfRange);
   };
   // B tile rows are stored with bit 4 of the column flipped on odd rows, so the
   // four rows an MFMA B fragment reads fall in different LDS bank halves.
   auto store_b = [&](int buf) {
 #pragma unroll
-    for (int h = 0; h < KB / 8; ++h) {
-      const int k = h * 8 + half;
+    for (int h = 0; h < KB / RP; ++h) {
+      const int k = h * RP + bk;
       *reinterpret_cast<double2*>(&sB[buf][k][c2 ^ ((k & 1) << 4)]) = bst[h];
     }
   };
@@ -544,8 +548,10 @@ __device__ __forceinline__ void group_tiles(int T0, int T1, int grp, int ngroups
   jt1 = T0 + ((grp + 1) * (T1 - T0)) / ngroups;
 }
 
-template <int KB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_update(
+// NBANDS = 1: 256 threads, one band per workgroup; NBANDS = 2: 512 threads, two
+// bands sharing each B tile (half the B-row traffic).
+template <int KB, int NBANDS>
+__global__ __launch_bounds__(256 * NBANDS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_update(
     const double* __restrict__ X, double* __restrict__ Out, long long ostride, int ldo, int ocol0, int Np, int k0,
     const double* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1,
     int ngroups) {
@@ -553,14 +559,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   int jt0, jt1;
   group_tiles(T0, T1, blockIdx.x % ngroups, ngroups, jt0, jt1);
   if (jt0 >= jt1) return;
-  gj_update_body<KB>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, blockIdx.y, blockIdx.x / ngroups, jt0, jt1,
-                     threadIdx.x, sB);
+  gj_update_body<KB, NBANDS>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, blockIdx.y,
+                             NBANDS * (blockIdx.x / ngroups), jt0, jt1, threadIdx.x, sB);
 }
 
 // One launch with two jobs on disjoint matrices' data: the first `npanel`
 // workgroups factor the next pair's panel (one matrix each, as k_gj_panel), the
-// rest run column tiles [T0, T1) of the current pair's rank-2NB update as two
-// independent 256-thread halves (two 64-row bands).  Workgroups dispatch in
+// rest run column tiles [T0, T1) of the current pair's rank-2NB update, two
+// 64-row bands per workgroup.  Workgroups dispatch in
 // index order, so the latency-bound panels take their CUs first and the update
 // fills the others -- the panel chain of pair q+1 hides behind update q.
 template <int RPT>
@@ -576,17 +582,17 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                             prev_src, prev_piv, prev_k0, psrc2, pbrow, blockIdx.x);
     return;
   }
-  __shared__ double sB[2][2][2 * NB][64];
+  __shared__ double sB[2][2 * NB][64];
   const int u = blockIdx.x - npanel;
   const int band_pairs = ((Np + 63) / 64 + 1) / 2;
   const int per_b = band_pairs * ngroups;
   const int b = u / per_b, rem = u % per_b;
-  const int half = threadIdx.x >> 8;  // a half whose band lies past Np only joins the barriers
   int jt0, jt1;
   group_tiles(T0, T1, rem % ngroups, ngroups, jt0, jt1);
   if (jt0 >= jt1) return;
-  gj_update_body<2 * NB>(X, Out, (long long)Np * Np, Np, 0, Np, k0, AT, src2, brow, b, 2 * (rem / ngroups) + half,
-                         jt0, jt1, threadIdx.x & 255, sB[half]);
+  // a band past Np (odd band count) only stages B tiles and joins the barriers
+  gj_update_body<2 * NB, 2>(X, Out, (long long)Np * Np, Np, 0, Np, k0, AT, src2, brow, b, 2 * (rem / ngroups), jt0,
+                            jt1, threadIdx.x, sB);
 }
 
 __global__ void k_unpermute(const double* __restrict__ in, double* __restrict__ out, int Np,
@@ -680,10 +686,13 @@ template <int KB>
 int launch_update(const double* cur, double* out, long long ostride, int ldo, int ocol0, int batch, int Np, int k0,
                   const double* AT, const int* src2, const int* brow, int T0, int T1, bool sample,
                   hipStream_t stream) {
-  const int bands = (Np + 63) / 64;
+  // standalone launches run one band per workgroup (faster alone than the
+  // two-band form, which pays off beside a panel in k_gj_fused)
+  constexpr int NBANDS = 1;
+  const int units = ((Np + 63) / 64 + NBANDS - 1) / NBANDS;  // band groups of NBANDS bands
   const int span = T1 - T0;
-  // enough workgroups to fill the chip twice over; each walks span/ngroups tiles
-  int ngroups = (2048 + bands * batch - 1) / (bands * batch);
+  // about 2048 / NBANDS workgroups: each walks span/ngroups tiles
+  int ngroups = (2048 / NBANDS + units * batch - 1) / (units * batch);
   if (ngroups > span) ngroups = span;
   if (ngroups < 1) ngroups = 1;
   // live timing covers the rank-2NB launches (the sweep's dominant kernel)
@@ -694,8 +703,8 @@ int launch_update(const double* cur, double* out, long long ostride, int ldo, in
     g_prof.batch[slot] = batch * (KB / NB);  // block steps performed by this launch
     TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
   }
-  hipLaunchKernelGGL((k_gj_update<KB>), dim3(bands * ngroups, batch), dim3(256), 0, stream, cur, out, ostride, ldo,
-                     ocol0, Np, k0, AT, src2, brow, T0, T1, ngroups);
+  hipLaunchKernelGGL((k_gj_update<KB, NBANDS>), dim3(units * ngroups, batch), dim3(256 * NBANDS), 0, stream, cur, out,
+                     ostride, ldo, ocol0, Np, k0, AT, src2, brow, T0, T1, ngroups);
   TICC_CHECK_LAUNCH();
   if (sample) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
   return TICC_OK;
