@@ -1,24 +1,26 @@
 // Batched blocked Gauss-Jordan inversion with partial pivoting (gfx950).
 //
 // A batch of padded Np x Np FP64 matrices is inverted in Np / NB block steps
-// of NB columns, taken two at a time (k, k+1 -> one rank-2NB update):
-//   1. k_gj_panel (one workgroup per matrix): in-place Gauss-Jordan on the
-//      Np x NB column panel of step k, pivot rows chosen by partial pivoting
-//      among the rows not yet used (LAPACK idamax order: largest |a|, first
-//      position on ties).  The panel lives in registers, up to 2 rows per
-//      thread.  Outputs the transformed panel P_k in output-row order, the
-//      row source map s_k and the NB pivot rows p_k.
-//   2. k_panel_gemm: the columns of step k+1 after step k,
-//        Z[i] = mask_k(i) X[s_k(i)][cols_{k+1}] + P_k[i] . X[p_k][cols_{k+1}],
-//      then k_gj_panel on Z gives P_{k+1}, s_{k+1}, p_{k+1}, the combined row
-//      map and the 2NB rows of X that the pair multiplies.
-//   3. k_panel_gemm again: combined step-k coefficients
-//        A_k[i] = mask_{k+1}(i) P_k[s_{k+1}(i)] + P_{k+1}[i] . P_k[p_{k+1}].
-//   4. k_gj_update<2NB> (row bands x matrices): the rank-2NB update
-//        Out[i][:] = m(i) X[s_k(s_{k+1}(i))][:] + [A_k | P_{k+1}][i] . [X[p_k]; X[s_k(p_{k+1})]]
-//      -- one pass over X for two Gauss-Jordan steps, FP64 MFMA, reading X and
-//      writing the ping-pong partner buffer; the panel columns are replaced by
-//      the coefficients.  An odd last step uses the rank-NB form.
+// of NB columns, taken two at a time (steps k, k+1 -> one rank-2NB update).
+// The panel chain of a pair runs in one workgroup per matrix
+// (gj_pair_chain_body):
+//   1. panel of step k: in-place Gauss-Jordan on the Np x NB column panel,
+//      pivot rows chosen by partial pivoting among the rows not yet used
+//      (LAPACK idamax order: largest |a|, first position on ties), the panel
+//      in registers, up to 2 rows per thread -> transformed panel P_k in
+//      output-row order, row source map s_k, pivot rows p_k;
+//   2. Z[i] = mask_k(i) X[s_k(i)][cols_{k+1}] + P_k[i] . X[p_k][cols_{k+1}]
+//      (the columns of step k+1 after step k, FP64 MFMA);
+//   3. panel of Z -> P_{k+1}, s_{k+1}, p_{k+1}, the combined row map, the 2NB
+//      rows of X the pair multiplies and the transposed coefficients;
+//   4. A_k[i] = mask_{k+1}(i) P_k[s_{k+1}(i)] + P_{k+1}[i] . P_k[p_{k+1}].
+// Then k_gj_update<2NB> applies
+//   Out[i][:] = m(i) X[s_k(s_{k+1}(i))][:] + [A_k | P_{k+1}][i] . [X[p_k]; X[s_k(p_{k+1})]]
+// -- one pass over X for two Gauss-Jordan steps, FP64 MFMA, reading X and
+// writing the ping-pong partner buffer; the panel columns are replaced by the
+// coefficients.  An odd last step uses the rank-NB form.  The panel chain of
+// pair q+1 runs inside the update of pair q (k_gj_fused); it needs only a
+// one-tile look-ahead of that update.
 // Row swaps compose into perm (original row at each position); the inverse of
 // the matrix is Out with its columns gathered through pinv = perm^-1, which the
 // consumers fuse into their own reads.
@@ -299,19 +301,19 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
                               prev_piv, prev_k0, src2, brow, blockIdx.x);
 }
 
-// Small panel GEMM (FP64 MFMA) of the paired block step:
+// Small panel GEMM (FP64 MFMA) of the paired block step, for one matrix b:
 //   Out[i][0:NB] = mask(i) C[gC(i)][colC : colC+NB] + A[i][0:NB] . B[gB(r)][colB : colB+NB],
 //   mask(i) = 0 for i in [mlo, mlo+NB), 1 otherwise.
 // Out is row-major (Np x NB) or, with `ostride` > 0, transposed (NB rows of
-// length Np, matrix stride ostride).  One workgroup (4 waves) per 64 rows of one
-// matrix, each wave 16 rows x NB.
-__global__ __launch_bounds__(256) void k_panel_gemm(const double* __restrict__ A, const double* __restrict__ C,
-                                                    long long cb, int ldC, int colC, const int* __restrict__ gC,
-                                                    const double* __restrict__ Bm, long long bb, int ldB, int colB,
-                                                    const int* __restrict__ gB, int mlo, int Np,
-                                                    double* __restrict__ Out, long long ostride) {
-  __shared__ double sB[NB][NB + 1];
-  const int b = blockIdx.y;
+// length Np, matrix stride ostride).  The workgroup's waves take 16-row blocks
+// in turn; sB stages the NB gathered B rows.
+template <int THREADS>
+__device__ __forceinline__ void panel_gemm_body(const double* __restrict__ A, const double* __restrict__ C,
+                                                long long cb, int ldC, int colC, const int* __restrict__ gC,
+                                                const double* __restrict__ Bm, long long bb, int ldB, int colB,
+                                                const int* __restrict__ gB, int mlo, int Np,
+                                                double* __restrict__ Out, long long ostride, int b,
+                                                double (*sB)[NB + 1]) {
   const int t = threadIdx.x;
   const int wave = t >> 6, lane = t & 63, lr = lane & 15, lq = lane >> 4;
   const double* Ab = A + (size_t)b * Np * NB;
@@ -319,51 +321,84 @@ __global__ __launch_bounds__(256) void k_panel_gemm(const double* __restrict__ A
   const double* Bb = Bm + (size_t)b * bb;
   const int* gCb = gC + (size_t)b * Np;
   const int* gBb = gB + b * NB;
-  {  // stage B: NB gathered rows x NB columns
-    const int r = t >> 3, seg = t & 7;  // 32 rows x 8 segments of 4
-    const double* br = Bb + (size_t)gBb[r] * ldB + colB + seg * 4;
-#pragma unroll
-    for (int h = 0; h < 4; ++h) sB[r][seg * 4 + h] = br[h];
-  }
-  const int row0 = blockIdx.x * 64 + wave * 16;
-  const bool active = row0 < Np;
-  double4_t acc[2];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int row = active ? row0 + mfma_drow(lq, r) : 0;
-    const bool m = (row >= mlo) && (row < mlo + NB);
-    const double* cr = Cb + (size_t)gCb[row] * ldC + colC + lr;
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj) {
-      const double cv = cr[16 * bj];
-      acc[bj][r] = m ? 0.0 : cv;
-    }
+  for (int e = t; e < NB * NB; e += THREADS) {
+    const int r = e / NB, cc = e % NB;
+    sB[r][cc] = Bb[(size_t)gBb[r] * ldB + colB + cc];
   }
   __syncthreads();
-  if (!active) return;
-#pragma unroll
-  for (int kk = 0; kk < NB / 4; ++kk) {
-    const int kq = 4 * kk + lq;
-    const double a = Ab[(size_t)(row0 + lr) * NB + kq];
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-      acc[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sB[kq][16 * bj + lr], acc[bj], 0, 0, 0);
-  }
-  if (ostride > 0) {
-    double* Ob = Out + (size_t)b * ostride;
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj) Ob[(size_t)(16 * bj + lr) * Np + row0 + mfma_drow(lq, r)] = acc[bj][r];
-  } else {
-    double* Ob = Out + (size_t)b * Np * NB;
+  for (int row0 = wave * 16; row0 < Np; row0 += THREADS / 4) {
+    double4_t acc[2];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = row0 + mfma_drow(lq, r);
+      const bool m = (row >= mlo) && (row < mlo + NB);
+      const double* cr = Cb + (size_t)gCb[row] * ldC + colC + lr;
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj) Ob[(size_t)row * NB + 16 * bj + lr] = acc[bj][r];
+      for (int bj = 0; bj < 2; ++bj) {
+        const double cv = cr[16 * bj];
+        acc[bj][r] = m ? 0.0 : cv;
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < NB / 4; ++kk) {
+      const int kq = 4 * kk + lq;
+      const double a = Ab[(size_t)(row0 + lr) * NB + kq];
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+        acc[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sB[kq][16 * bj + lr], acc[bj], 0, 0, 0);
+    }
+    if (ostride > 0) {
+      double* Ob = Out + (size_t)b * ostride;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) Ob[(size_t)(16 * bj + lr) * Np + row0 + mfma_drow(lq, r)] = acc[bj][r];
+    } else {
+      double* Ob = Out + (size_t)b * Np * NB;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = row0 + mfma_drow(lq, r);
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) Ob[(size_t)row * NB + 16 * bj + lr] = acc[bj][r];
+      }
     }
   }
+}
+
+// The whole panel chain of one pair of steps (k0, k0+NB) for matrix b, by one
+// workgroup: panel of step k0 read from the columns [xcol, xcol+NB) of Xc;
+// Z = the columns of step k0+NB after step k0; panel of Z (also writing the
+// transposed coefficients and the pair's row maps); the combined step-k0
+// coefficients, written transposed to AT[0:NB].
+struct PairJob {
+  const double* Xc;
+  long long xs;
+  int ldx, xcol, Np, k0;
+  double *P1, *P2, *Z, *AT;
+  int *s1, *s2, *p1, *p2, *src2, *brow, *perm, *pinv;
+};
+
+template <int RPT>
+__device__ __forceinline__ void gj_pair_chain_body(const PairJob& P, int b) {
+  __shared__ double sG[NB][NB + 1];
+  const int Np = P.Np, k0 = P.k0, k1 = P.k0 + NB;
+  const long long pn = (long long)Np * NB;
+  const long long at_stride = 2LL * NB * Np;
+  gj_panel_body<512, RPT>(P.Xc, P.xs, P.ldx, P.xcol, Np, k0, P.P1, P.s1, P.p1, P.perm, P.pinv, nullptr, 0, nullptr,
+                          nullptr, 0, nullptr, nullptr, b);
+  __syncthreads();
+  panel_gemm_body<512>(P.P1, P.Xc, P.xs, P.ldx, P.xcol + NB, P.s1, P.Xc, P.xs, P.ldx, P.xcol + NB, P.p1, k0, Np, P.Z,
+                       0LL, b, sG);
+  __syncthreads();
+  gj_panel_body<512, RPT>(P.Z, pn, NB, 0, Np, k1, P.P2, P.s2, P.p2, P.perm, P.pinv, P.AT + (size_t)NB * Np, at_stride,
+                          P.s1, P.p1, k0, P.src2, P.brow, b);
+  __syncthreads();
+  panel_gemm_body<512>(P.P2, P.P1, pn, NB, 0, P.s2, P.P1, pn, NB, 0, P.p2, k1, Np, P.AT, at_stride, b, sG);
+}
+
+template <int RPT>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_pair_chain(PairJob P) {
+  gj_pair_chain_body<RPT>(P, blockIdx.x);
 }
 
 // Raw buffer loads: an offset past the end of the buffer reads 0 without a
@@ -563,23 +598,19 @@ __global__ __launch_bounds__(256 * NBANDS) __attribute__((amdgpu_waves_per_eu(2,
                              NBANDS * (blockIdx.x / ngroups), jt0, jt1, threadIdx.x, sB);
 }
 
-// One launch with two jobs on disjoint matrices' data: the first `npanel`
-// workgroups factor the next pair's panel (one matrix each, as k_gj_panel), the
+// One launch with two jobs on disjoint data: the first `npanel` workgroups run
+// the panel chain of the next pair (one matrix each, as k_gj_pair_chain), the
 // rest run column tiles [T0, T1) of the current pair's rank-2NB update, two
-// 64-row bands per workgroup.  Workgroups dispatch in
-// index order, so the latency-bound panels take their CUs first and the update
-// fills the others -- the panel chain of pair q+1 hides behind update q.
+// 64-row bands per workgroup.  Workgroups dispatch in index order, so the
+// latency-bound panel chains take their CUs first and the update fills the
+// others -- the panel chain of pair q+1 hides behind update q.
 template <int RPT>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_fused(
-    const double* __restrict__ PX, long long pbstride, int pld, int pcol0, int pk0, double* __restrict__ Pout,
-    int* __restrict__ psrc, int* __restrict__ ppiv, int* __restrict__ perm, int* __restrict__ pinv,
-    double* __restrict__ PoutT, long long tstride, const int* __restrict__ prev_src,
-    const int* __restrict__ prev_piv, int prev_k0, int* __restrict__ psrc2, int* __restrict__ pbrow, int npanel,
-    const double* __restrict__ X, double* __restrict__ Out, int Np, int k0, const double* __restrict__ AT,
-    const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1, int ngroups) {
+    PairJob P, int npanel, const double* __restrict__ X, double* __restrict__ Out, int Np, int k0,
+    const double* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1,
+    int ngroups) {
   if ((int)blockIdx.x < npanel) {
-    gj_panel_body<512, RPT>(PX, pbstride, pld, pcol0, Np, pk0, Pout, psrc, ppiv, perm, pinv, PoutT, tstride,
-                            prev_src, prev_piv, prev_k0, psrc2, pbrow, blockIdx.x);
+    gj_pair_chain_body<RPT>(P, blockIdx.x);
     return;
   }
   __shared__ double sB[2][2 * NB][64];
@@ -737,32 +768,35 @@ Carve carve(double* pout, int* src, int* piv, int batch, int Np) {
   return c;
 }
 
-// Panels and coefficients of the pair of steps (k0, k0+NB), read from the
-// panel columns held in Xc (matrix stride xs, leading dimension ldx, the
-// pair's first column at xcol); AT, src2 and brow receive the update operands.
-int pair_chain(const double* Xc, long long xs, int ldx, int xcol, int batch, int Np, int k0, const Carve& c,
-               double* AT, int* src2, int* brow, int* perm, int* pinv, hipStream_t st) {
-  const int k1 = k0 + NB;
-  const int rtiles = (Np + 63) / 64;
-  const long long pn = (long long)Np * NB;
-  const long long at_stride = 2LL * NB * Np;
-  TICC_RETURN_IF(launch_panel(Xc, xs, ldx, xcol, batch, Np, k0, c.P1, c.s1, c.p1, perm, pinv, PanelMaps{}, st));
-  // columns of step k+1 after step k: Z = X_{k+1}[:, k1:k1+NB]
-  hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, st, c.P1, Xc, xs, ldx, xcol + NB, c.s1, Xc, xs,
-                     ldx, xcol + NB, c.p1, k0, Np, c.Z, 0LL);
-  TICC_CHECK_LAUNCH();
-  PanelMaps m;
-  m.PoutT = AT + (size_t)NB * Np;
-  m.tstride = at_stride;
-  m.prev_src = c.s1;
-  m.prev_piv = c.p1;
-  m.prev_k0 = k0;
-  m.src2 = src2;
-  m.brow = brow;
-  TICC_RETURN_IF(launch_panel(c.Z, pn, NB, 0, batch, Np, k1, c.P2, c.s2, c.p2, perm, pinv, m, st));
-  // combined step-k coefficients A1 = mask P1[s2] + P2 . P1[p2], stored transposed in AT[0:NB]
-  hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, st, c.P2, c.P1, pn, NB, 0, c.s2, c.P1, pn, NB,
-                     0, c.p2, k1, Np, AT, at_stride);
+PairJob pair_job(const double* Xc, long long xs, int ldx, int xcol, int Np, int k0, const Carve& c, int par,
+                 int* perm, int* pinv) {
+  PairJob P;
+  P.Xc = Xc;
+  P.xs = xs;
+  P.ldx = ldx;
+  P.xcol = xcol;
+  P.Np = Np;
+  P.k0 = k0;
+  P.P1 = c.P1;
+  P.P2 = c.P2;
+  P.Z = c.Z;
+  P.AT = c.AT[par];
+  P.s1 = c.s1;
+  P.s2 = c.s2;
+  P.p1 = c.p1;
+  P.p2 = c.p2;
+  P.src2 = c.src2[par];
+  P.brow = c.brow[par];
+  P.perm = perm;
+  P.pinv = pinv;
+  return P;
+}
+
+int launch_pair_chain(const PairJob& P, int batch, hipStream_t stream) {
+  if (P.Np <= 512)
+    hipLaunchKernelGGL((k_gj_pair_chain<1>), dim3(batch), dim3(512), 0, stream, P);
+  else
+    hipLaunchKernelGGL((k_gj_pair_chain<2>), dim3(batch), dim3(512), 0, stream, P);
   TICC_CHECK_LAUNCH();
   return TICC_OK;
 }
@@ -779,11 +813,11 @@ int single_step(double* cur, double* nxt, int batch, int Np, int k0, const Carve
   return launch_update<NB>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[0], c.src2[0], c.brow[0], 0, (Np + 63) / 64,
                            sample, st);
 }
-// Fused launch: the panel of step pk0 (reading columns [pcol0, pcol0+NB) of PX)
-// beside column tiles [T0, T1) of the rank-2NB update of the pair at k0.
-int launch_fused(const double* PX, long long pbstride, int pld, int pcol0, int pk0, double* pout, int* psrc,
-                 int* ppiv, int* perm, int* pinv, const PanelMaps& m, const double* X, double* Out, int batch, int Np,
-                 int k0, const double* AT, const int* src2, const int* brow, int T0, int T1, hipStream_t stream) {
+
+// Fused launch: the panel chain P beside column tiles [T0, T1) of the rank-2NB
+// update of the pair at k0.
+int launch_fused(const PairJob& P, const double* X, double* Out, int batch, int Np, int k0, const double* AT,
+                 const int* src2, const int* brow, int T0, int T1, hipStream_t stream) {
   const int band_pairs = ((Np + 63) / 64 + 1) / 2;
   // about two workgroups per CU: longer tile walks amortise each band's
   // prologue (A fragments, first C and B tiles) better than more, shorter ones
@@ -791,14 +825,12 @@ int launch_fused(const double* PX, long long pbstride, int pld, int pcol0, int p
   if (ngroups > T1 - T0) ngroups = T1 - T0;
   if (ngroups < 1) ngroups = 1;
   const int grid = batch + band_pairs * ngroups * batch;
-#define TICC_FUSED_ARGS                                                                                            \
-  PX, pbstride, pld, pcol0, pk0, pout, psrc, ppiv, perm, pinv, m.PoutT, m.tstride, m.prev_src, m.prev_piv,         \
-      m.prev_k0, m.src2, m.brow, batch, X, Out, Np, k0, AT, src2, brow, T0, T1, ngroups
   if (Np <= 512)
-    hipLaunchKernelGGL((k_gj_fused<1>), dim3(grid), dim3(512), 0, stream, TICC_FUSED_ARGS);
+    hipLaunchKernelGGL((k_gj_fused<1>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
+                       T0, T1, ngroups);
   else
-    hipLaunchKernelGGL((k_gj_fused<2>), dim3(grid), dim3(512), 0, stream, TICC_FUSED_ARGS);
-#undef TICC_FUSED_ARGS
+    hipLaunchKernelGGL((k_gj_fused<2>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
+                       T0, T1, ngroups);
   TICC_CHECK_LAUNCH();
   return TICC_OK;
 }
@@ -820,11 +852,8 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
   }
   const int steps = Np / NB;
   const int pairs = steps / 2;
-  const long long pn = (long long)Np * NB;
   const long long ln = 2LL * NB * Np;  // look-ahead buffer: Np x 2NB per matrix
-  const int rtiles = (Np + 63) / 64;
-  if (pairs > 0) TICC_RETURN_IF(pair_chain(buf[0], n2, Np, 0, batch, Np, 0, c, c.AT[0], c.src2[0], c.brow[0], perm,
-                                           pinv, stream));
+  if (pairs > 0) TICC_RETURN_IF(launch_pair_chain(pair_job(buf[0], n2, Np, 0, Np, 0, c, 0, perm, pinv), batch, stream));
   for (int q = 0; q < pairs; ++q) {
     const int k0 = 2 * q * NB;
     const int par = q & 1;
@@ -835,30 +864,13 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
                                            0, tiles, sample, stream));
       break;
     }
-    // Pair q+1 (columns kn, kn + NB) is factored while update q runs: its
-    // columns after pair q come from a one-tile look-ahead of update q.
+    // The panel chain of pair q+1 (columns kn, kn + NB) runs inside update q:
+    // its columns after pair q come from a one-tile look-ahead of update q.
     const int kn = k0 + 2 * NB;
     TICC_RETURN_IF(launch_update<2 * NB>(cur, c.L, ln, 2 * NB, kn, batch, Np, k0, c.AT[par], c.src2[par],
                                          c.brow[par], kn / 64, kn / 64 + 1, false, stream));
-    const int th = tiles / 2;
-    TICC_RETURN_IF(launch_fused(c.L, ln, 2 * NB, 0, kn, c.P1, c.s1, c.p1, perm, pinv, PanelMaps{}, cur, nxt, batch,
-                                Np, k0, c.AT[par], c.src2[par], c.brow[par], 0, th, stream));
-    hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, stream, c.P1, c.L, ln, 2 * NB, NB, c.s1, c.L,
-                       ln, 2 * NB, NB, c.p1, kn, Np, c.Z, 0LL);
-    TICC_CHECK_LAUNCH();
-    PanelMaps m;
-    m.PoutT = c.AT[par ^ 1] + (size_t)NB * Np;
-    m.tstride = 2LL * NB * Np;
-    m.prev_src = c.s1;
-    m.prev_piv = c.p1;
-    m.prev_k0 = kn;
-    m.src2 = c.src2[par ^ 1];
-    m.brow = c.brow[par ^ 1];
-    TICC_RETURN_IF(launch_fused(c.Z, pn, NB, 0, kn + NB, c.P2, c.s2, c.p2, perm, pinv, m, cur, nxt, batch, Np, k0,
-                                c.AT[par], c.src2[par], c.brow[par], th, tiles, stream));
-    hipLaunchKernelGGL(k_panel_gemm, dim3(rtiles, batch), dim3(256), 0, stream, c.P2, c.P1, pn, NB, 0, c.s2, c.P1, pn,
-                       NB, 0, c.p2, kn + NB, Np, c.AT[par ^ 1], 2LL * NB * Np);
-    TICC_CHECK_LAUNCH();
+    TICC_RETURN_IF(launch_fused(pair_job(c.L, ln, 2 * NB, 0, Np, kn, c, par ^ 1, perm, pinv), cur, nxt, batch, Np, k0,
+                                c.AT[par], c.src2[par], c.brow[par], 0, tiles, stream));
   }
   double* cur = buf[pairs & 1];
   if (steps & 1) {  // trailing single step: rank-NB update
