@@ -8,14 +8,16 @@ basis rows resident in HBM: W(R) contraction at every radial point, the FP64
 LDMD sweep of all 64 energies, and the fused BF->SF + K->S matching.
 
 value = energies x sectors / seconds per step (energy.R-steps/s).
-roofline: the Gauss-Jordan update kernel k_gj_update (rank 64 = two block
-steps per launch; the sweep's dominant kernel), timed live with HIP events on
-the stream it runs on.  Per launch on m matrices: algorithmic bytes
-m (2 Np^2 + 64 Np) 8 (read X, write the partner buffer, read the coefficient
-columns) and FP64 flop 2 Np^2 64 m -- 8 flop/byte, below the FP64 ridge point,
-so the bound is HBM; the MFMA rate is reported beside it.  traffic: HBM bytes
-per 64-matrix launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-summary (profiles/), when it was measured on this workload.
+roofline: the sweep's dominant kernel, k_gj_fused -- the rank-64 Gauss-Jordan
+update of a pair of block steps (FP64 MFMA) with the next pair's panel chain in
+its first workgroups -- timed live with HIP events on the stream it runs on.
+Per launch on m matrices: algorithmic bytes m (2 Np^2 + 64 Np + 128 Np) 8 (the
+update reads X, writes the partner buffer and reads the coefficient columns;
+the panel chain reads its look-ahead columns and writes its coefficients) and
+FP64 flop 2 Np^2 64 m -- about 7.7 flop/byte, below the FP64 ridge point, so
+the bound is HBM; the MFMA rate is reported beside it.  traffic: HBM bytes per
+64-matrix launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+(profiles/), when it was measured on this workload.
 cpu_baseline: the NumPy oracle (a restatement of the reference algorithm) on
 the same system, all 64 energies over the first sectors until >= 10 s of
 work, on this host's cores.
@@ -85,7 +87,7 @@ def cpu_baseline(workload, grid, min_seconds: float = 10.0, max_sectors: int = 6
 
 def _traffic(n_channel: int, n_energy: int):
     """HBM bytes per rank-64 update launch from the committed counter summary, if it matches."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_gj_update_rotor_rotor_N800_traffic.json")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_gj_fused_rotor_rotor_N800_traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)
@@ -102,7 +104,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-every", type=int, default=8, help="time every k-th batched inversion's update launches")
+    ap.add_argument("--profile-every", type=int, default=8, help="time every k-th batched inversion's fused launches")
     ap.add_argument("--max-sectors", type=int, default=0, help="propagate only the first M sectors (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -152,24 +154,23 @@ def main() -> int:
     elapsed = time.perf_counter() - t
     tot_ms = np.zeros(1)
     cnt = np.zeros(1, dtype=np.int32)
-    nbat = np.zeros(1, dtype=np.int64)
+    tot_bytes = np.zeros(1)
+    tot_flop = np.zeros(1)
     _backend.check(lib.ticc_profile_update_end(tot_ms.ctypes.data_as(_backend._dp), cnt.ctypes.data_as(_backend._ip),
-                                               nbat.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
+                                               tot_bytes.ctypes.data_as(_backend._dp),
+                                               tot_flop.ctypes.data_as(_backend._dp)))
     ms_per_step = 1e3 * elapsed / args.steps
     value = nE * nS / (elapsed / args.steps)
 
-    Np = (N + 31) // 32 * 32
-    steps_x_matrices = float(nbat[0]) / max(1, int(cnt[0]))  # block steps x matrices per timed launch
-    matrices = steps_x_matrices / 2.0                          # rank-64 launches: two block steps
-    avg_ms = float(tot_ms[0]) / max(1, int(cnt[0]))
-    flop_per_launch = 2.0 * Np * Np * 32 * steps_x_matrices
-    # algorithmic HBM bytes: read X, write the partner buffer, read the 64 coefficient columns
-    bytes_per_launch = matrices * (2.0 * Np * Np * 8 + 64.0 * Np * 8)
-    achieved_gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if cnt[0] else None
-    achieved_tf = flop_per_launch / (avg_ms * 1e-3) / 1e12 if cnt[0] else None
+    n_timed = int(cnt[0])
+    avg_ms = float(tot_ms[0]) / max(1, n_timed)
+    bytes_per_launch = float(tot_bytes[0]) / max(1, n_timed)
+    flop_per_launch = float(tot_flop[0]) / max(1, n_timed)
+    achieved_gbs = float(tot_bytes[0]) / (float(tot_ms[0]) * 1e-3) / 1e9 if n_timed else None
+    achieved_tf = float(tot_flop[0]) / (float(tot_ms[0]) * 1e-3) / 1e12 if n_timed else None
     S = unpack_smat(S_re, S_im, n_open)
     defect = max(float(np.max(np.abs(s.conj().T @ s - np.eye(s.shape[0])))) for s in S if s.size)
-    _log(f"update kernel: {int(cnt[0])} timed launches, avg {avg_ms * 1e3:.1f} us; unitarity defect {defect:.2e}")
+    _log(f"fused kernel: {n_timed} timed launches, avg {avg_ms * 1e3:.1f} us; unitarity defect {defect:.2e}")
 
     traffic = _traffic(N, nE)
     cpu = None
@@ -200,8 +201,9 @@ def main() -> int:
             "precision": "fp64",
         },
         "roofline": {
-            "kernel": "k_gj_update<64> (rank-64 Gauss-Jordan update, two block steps per launch, FP64 MFMA)",
-            # 8 flop per algorithmic byte, below the MI355X FP64 ridge point (78.6 TF / 8 TB/s = 9.8)
+            "kernel": "k_gj_fused<2> (rank-64 Gauss-Jordan update of a pair of block steps, FP64 MFMA, beside the "
+            "next pair's panel chain)",
+            # 7.7 flop per algorithmic byte, below the MI355X FP64 ridge point (78.6 TF / 8 TB/s = 9.8)
             "bound": "hbm",
             "achieved": achieved_gbs,
             "peak": HBM_PEAK_GBS,
@@ -210,8 +212,7 @@ def main() -> int:
             "traffic": traffic,
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_us": avg_ms * 1e3,
-            "matrices_per_launch": matrices,
-            "timed_launches": int(cnt[0]),
+            "timed_launches": n_timed,
             "mfma_tflops": achieved_tf,
             "mfma_frac_of_fp64_peak": None if achieved_tf is None else achieved_tf / FP64_MFMA_PEAK_TFLOPS,
         },

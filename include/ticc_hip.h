@@ -158,15 +158,15 @@ int ticc_dgemm_batched(int m, int n, int k, double alpha, const double* A, long 
 
 /*
  * Instrumentation (no counterpart in the reference): live HIP-event timing of
- * the rank-64 Gauss-Jordan update kernel (two block steps per launch), the
- * sweep's dominant kernel.  Between begin and end, every sample_every-th
- * batched inversion has each of its rank-64 update launches bracketed by an
- * event pair on the stream it runs on (at most max_samples pairs).  end
- * synchronises the device and returns the summed kernel time, the number of
- * timed launches and the summed (block steps x matrices) they performed.
+ * the sweep's dominant kernel, the fused Gauss-Jordan launch (one pair's
+ * rank-64 update beside the next pair's panel chain).  Between begin and end,
+ * every sample_every-th batched inversion has each of its fused launches
+ * bracketed by an event pair on the stream it runs on (at most max_samples
+ * pairs).  end synchronises the device and returns the summed kernel time, the
+ * number of timed launches and their summed algorithmic bytes and FP64 flop.
  */
 int ticc_profile_update_begin(int sample_every, int max_samples);
-int ticc_profile_update_end(double* total_ms, int* count, long long* total_batch);
+int ticc_profile_update_end(double* total_ms, int* count, double* total_bytes, double* total_flop);
 
 #ifdef __cplusplus
 }

@@ -44,7 +44,7 @@ _SIGNATURES = {
     "ticc_bessel_eval": ([_c_int, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "ticc_invert_batched": ([_c_int, _c_int, _vp, _vp], _c_int),
     "ticc_profile_update_begin": ([_c_int, _c_int], _c_int),
-    "ticc_profile_update_end": ([_dp, _ip, ctypes.POINTER(ctypes.c_longlong)], _c_int),
+    "ticc_profile_update_end": ([_dp, _ip, _dp, _dp], _c_int),
     "ticc_dgemm_batched": ([_c_int, _c_int, _c_int, _c_d, _vp, _c_ll, _vp, _c_ll, _c_d, _vp, _c_ll, _c_int, _vp], _c_int),
 }
 

@@ -16,7 +16,7 @@ int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, con
              const double* Eint_host, const double* L_host, double* S_re, double* S_im, int* n_open,
              hipStream_t stream);
 int profile_begin(int every, int max_pairs);
-int profile_end(double* total_ms, int* count, long long* total_batch);
+int profile_end(double* total_ms, int* count, double* total_bytes, double* total_flop);
 int bessel_eval(int n, const double* ell, const double* x, double* out, const double* nu, double* kout,
                 hipStream_t stream);
 }  // namespace ticc
@@ -178,9 +178,9 @@ int ticc_profile_update_begin(int sample_every, int max_samples) {
   return ticc::profile_begin(sample_every, max_samples);
 }
 
-int ticc_profile_update_end(double* total_ms, int* count, long long* total_batch) {
-  if (!total_ms || !count || !total_batch) return fail(TICC_ERR_VALUE, "profile: null pointer argument");
-  return ticc::profile_end(total_ms, count, total_batch);
+int ticc_profile_update_end(double* total_ms, int* count, double* total_bytes, double* total_flop) {
+  if (!total_ms || !count || !total_bytes || !total_flop) return fail(TICC_ERR_VALUE, "profile: null pointer argument");
+  return ticc::profile_end(total_ms, count, total_bytes, total_flop);
 }
 
 }  // extern "C"

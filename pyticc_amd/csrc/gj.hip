@@ -374,6 +374,7 @@ struct PairJob {
   const double* Xc;
   long long xs;
   int ldx, xcol, Np, k0;
+  int single;  // 1: only the panel of the trailing single step (with its update maps)
   double *P1, *P2, *Z, *AT;
   int *s1, *s2, *p1, *p2, *src2, *brow, *perm, *pinv;
 };
@@ -384,6 +385,11 @@ __device__ __forceinline__ void gj_pair_chain_body(const PairJob& P, int b) {
   const int Np = P.Np, k0 = P.k0, k1 = P.k0 + NB;
   const long long pn = (long long)Np * NB;
   const long long at_stride = 2LL * NB * Np;
+  if (P.single) {
+    gj_panel_body<512, RPT>(P.Xc, P.xs, P.ldx, P.xcol, Np, k0, P.P1, P.s1, P.p1, P.perm, P.pinv, P.AT, at_stride,
+                            nullptr, nullptr, 0, P.src2, P.brow, b);
+    return;
+  }
   gj_panel_body<512, RPT>(P.Xc, P.xs, P.ldx, P.xcol, Np, k0, P.P1, P.s1, P.p1, P.perm, P.pinv, nullptr, 0, nullptr,
                           nullptr, 0, nullptr, nullptr, b);
   __syncthreads();
@@ -637,15 +643,17 @@ __global__ void k_unpermute(const double* __restrict__ in, double* __restrict__ 
   }
 }
 
-// Live timing of the rank-2NB update kernel with HIP events: every `every`-th
-// inversion has each of its rank-2NB update launches bracketed by an event pair.
+// Live timing of the dominant kernel (k_gj_fused: a pair's update beside the
+// next pair's panel chain) with HIP events: every `every`-th inversion has each
+// of its fused launches bracketed by an event pair, recorded with the launch's
+// algorithmic bytes and flop.
 struct UpdateProfile {
   std::mutex mu;
   bool on = false;
   int every = 1;
   long long calls = 0;
   std::vector<hipEvent_t> events;  // pairs: [2i] start, [2i+1] stop
-  std::vector<int> batch;          // matrices in each timed launch
+  std::vector<double> bytes, flop;  // per timed launch
   size_t used = 0;
 };
 UpdateProfile g_prof;
@@ -662,26 +670,28 @@ int profile_begin(int every, int max_pairs) {
     TICC_CHECK_HIP(hipEventCreate(&e));
     g_prof.events.push_back(e);
   }
-  g_prof.batch.resize(g_prof.events.size() / 2);
+  g_prof.bytes.resize(g_prof.events.size() / 2);
+  g_prof.flop.resize(g_prof.events.size() / 2);
   g_prof.on = true;
   return TICC_OK;
 }
 
-int profile_end(double* total_ms, int* count, long long* total_batch) {
+int profile_end(double* total_ms, int* count, double* total_bytes, double* total_flop) {
   std::lock_guard<std::mutex> lock(g_prof.mu);
   g_prof.on = false;
   TICC_CHECK_HIP(hipDeviceSynchronize());
-  double tot = 0.0;
-  long long nb = 0;
+  double tot = 0.0, nbytes = 0.0, nflop = 0.0;
   for (size_t i = 0; i < g_prof.used; ++i) {
-    nb += g_prof.batch[i];
+    nbytes += g_prof.bytes[i];
+    nflop += g_prof.flop[i];
     float ms = 0.0f;
     TICC_CHECK_HIP(hipEventElapsedTime(&ms, g_prof.events[2 * i], g_prof.events[2 * i + 1]));
     tot += ms;
   }
   *total_ms = tot;
   *count = (int)g_prof.used;
-  *total_batch = nb;
+  *total_bytes = nbytes;
+  *total_flop = nflop;
   g_prof.used = 0;
   return TICC_OK;
 }
@@ -715,8 +725,7 @@ int launch_panel(const double* X, long long bstride, int ld, int col0, int batch
 
 template <int KB>
 int launch_update(const double* cur, double* out, long long ostride, int ldo, int ocol0, int batch, int Np, int k0,
-                  const double* AT, const int* src2, const int* brow, int T0, int T1, bool sample,
-                  hipStream_t stream) {
+                  const double* AT, const int* src2, const int* brow, int T0, int T1, hipStream_t stream) {
   // standalone launches run one band per workgroup (faster alone than the
   // two-band form, which pays off beside a panel in k_gj_fused)
   constexpr int NBANDS = 1;
@@ -726,18 +735,9 @@ int launch_update(const double* cur, double* out, long long ostride, int ldo, in
   int ngroups = (2048 / NBANDS + units * batch - 1) / (units * batch);
   if (ngroups > span) ngroups = span;
   if (ngroups < 1) ngroups = 1;
-  // live timing covers the rank-2NB launches (the sweep's dominant kernel)
-  sample = sample && KB == 2 * NB;
-  size_t slot = 0;
-  if (sample) {
-    slot = g_prof.used++;
-    g_prof.batch[slot] = batch * (KB / NB);  // block steps performed by this launch
-    TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
-  }
   hipLaunchKernelGGL((k_gj_update<KB, NBANDS>), dim3(units * ngroups, batch), dim3(256 * NBANDS), 0, stream, cur, out,
                      ostride, ldo, ocol0, Np, k0, AT, src2, brow, T0, T1, ngroups);
   TICC_CHECK_LAUNCH();
-  if (sample) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
   return TICC_OK;
 }
 
@@ -777,6 +777,7 @@ PairJob pair_job(const double* Xc, long long xs, int ldx, int xcol, int Np, int 
   P.xcol = xcol;
   P.Np = Np;
   P.k0 = k0;
+  P.single = 0;
   P.P1 = c.P1;
   P.P2 = c.P2;
   P.Z = c.Z;
@@ -802,7 +803,7 @@ int launch_pair_chain(const PairJob& P, int batch, hipStream_t stream) {
 }
 
 int single_step(double* cur, double* nxt, int batch, int Np, int k0, const Carve& c, int* perm, int* pinv,
-                bool sample, hipStream_t st) {
+                hipStream_t st) {
   const long long n2 = (long long)Np * Np;
   PanelMaps m;
   m.PoutT = c.AT[0];
@@ -810,14 +811,13 @@ int single_step(double* cur, double* nxt, int batch, int Np, int k0, const Carve
   m.src2 = c.src2[0];
   m.brow = c.brow[0];
   TICC_RETURN_IF(launch_panel(cur, n2, Np, k0, batch, Np, k0, c.P1, c.s1, c.p1, perm, pinv, m, st));
-  return launch_update<NB>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[0], c.src2[0], c.brow[0], 0, (Np + 63) / 64,
-                           sample, st);
+  return launch_update<NB>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[0], c.src2[0], c.brow[0], 0, (Np + 63) / 64, st);
 }
 
 // Fused launch: the panel chain P beside column tiles [T0, T1) of the rank-2NB
 // update of the pair at k0.
 int launch_fused(const PairJob& P, const double* X, double* Out, int batch, int Np, int k0, const double* AT,
-                 const int* src2, const int* brow, int T0, int T1, hipStream_t stream) {
+                 const int* src2, const int* brow, int T0, int T1, bool sample, hipStream_t stream) {
   const int band_pairs = ((Np + 63) / 64 + 1) / 2;
   // about two workgroups per CU: longer tile walks amortise each band's
   // prologue (A fragments, first C and B tiles) better than more, shorter ones
@@ -825,6 +825,18 @@ int launch_fused(const PairJob& P, const double* X, double* Out, int batch, int 
   if (ngroups > T1 - T0) ngroups = T1 - T0;
   if (ngroups < 1) ngroups = 1;
   const int grid = batch + band_pairs * ngroups * batch;
+  size_t slot = 0;
+  if (sample) {
+    // algorithmic work: the update reads X and writes the partner buffer on its
+    // columns and reads the 2NB coefficient columns; the panel chain reads its
+    // 2NB (or NB) look-ahead columns and writes 2NB coefficient columns
+    const double cols = (double)((T1 * 64 < Np ? T1 * 64 : Np) - T0 * 64);
+    const double kp = P.single ? NB : 2 * NB;
+    slot = g_prof.used++;
+    g_prof.bytes[slot] = 8.0 * batch * (2.0 * Np * cols + 2.0 * NB * Np + 2.0 * kp * Np);
+    g_prof.flop[slot] = 2.0 * batch * Np * cols * 2 * NB;
+    TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
+  }
   if (Np <= 512)
     hipLaunchKernelGGL((k_gj_fused<1>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
                        T0, T1, ngroups);
@@ -832,6 +844,7 @@ int launch_fused(const PairJob& P, const double* X, double* Out, int batch, int 
     hipLaunchKernelGGL((k_gj_fused<2>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
                        T0, T1, ngroups);
   TICC_CHECK_LAUNCH();
+  if (sample) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
   return TICC_OK;
 }
 }  // namespace
@@ -848,7 +861,7 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
   if (g_prof.on) {
     std::lock_guard<std::mutex> lock(g_prof.mu);
     sample = g_prof.on && (g_prof.calls++ % g_prof.every == 0) &&
-             (g_prof.used + (size_t)(Np / NB) <= g_prof.events.size() / 2);
+             (g_prof.used + (size_t)(Np / NB) <= g_prof.bytes.size());
   }
   const int steps = Np / NB;
   const int pairs = steps / 2;
@@ -860,21 +873,36 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
     double* cur = buf[q & 1];
     double* nxt = buf[(q + 1) & 1];
     if (q + 1 == pairs) {
-      TICC_RETURN_IF(launch_update<2 * NB>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[par], c.src2[par], c.brow[par],
-                                           0, tiles, sample, stream));
-      break;
+      if (!(steps & 1)) {
+        TICC_RETURN_IF(launch_update<2 * NB>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[par], c.src2[par],
+                                             c.brow[par], 0, tiles, stream));
+        break;
+      }
+      // trailing single step: its panel (on a look-ahead of its columns) runs
+      // inside the last pair's update, then the rank-NB update
+      const int ks = k0 + 2 * NB;
+      TICC_RETURN_IF(launch_update<2 * NB>(cur, c.L, ln, 2 * NB, ks, batch, Np, k0, c.AT[par], c.src2[par],
+                                           c.brow[par], ks / 64, ks / 64 + 1, stream));
+      PairJob P = pair_job(c.L, ln, 2 * NB, 0, Np, ks, c, par ^ 1, perm, pinv);
+      P.single = 1;
+      TICC_RETURN_IF(
+          launch_fused(P, cur, nxt, batch, Np, k0, c.AT[par], c.src2[par], c.brow[par], 0, tiles, sample, stream));
+      TICC_RETURN_IF(launch_update<NB>(nxt, cur, n2, Np, 0, batch, Np, ks, c.AT[par ^ 1], c.src2[par ^ 1],
+                                       c.brow[par ^ 1], 0, tiles, stream));
+      *result = cur;
+      return TICC_OK;
     }
     // The panel chain of pair q+1 (columns kn, kn + NB) runs inside update q:
     // its columns after pair q come from a one-tile look-ahead of update q.
     const int kn = k0 + 2 * NB;
     TICC_RETURN_IF(launch_update<2 * NB>(cur, c.L, ln, 2 * NB, kn, batch, Np, k0, c.AT[par], c.src2[par],
-                                         c.brow[par], kn / 64, kn / 64 + 1, false, stream));
+                                         c.brow[par], kn / 64, kn / 64 + 1, stream));
     TICC_RETURN_IF(launch_fused(pair_job(c.L, ln, 2 * NB, 0, Np, kn, c, par ^ 1, perm, pinv), cur, nxt, batch, Np, k0,
-                                c.AT[par], c.src2[par], c.brow[par], 0, tiles, stream));
+                                c.AT[par], c.src2[par], c.brow[par], 0, tiles, sample, stream));
   }
   double* cur = buf[pairs & 1];
   if (steps & 1) {  // trailing single step: rank-NB update
-    TICC_RETURN_IF(single_step(cur, buf[(pairs + 1) & 1], batch, Np, (steps - 1) * NB, c, perm, pinv, sample, stream));
+    TICC_RETURN_IF(single_step(cur, buf[(pairs + 1) & 1], batch, Np, (steps - 1) * NB, c, perm, pinv, stream));
     cur = buf[(pairs + 1) & 1];
   }
   *result = cur;
