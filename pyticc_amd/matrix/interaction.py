@@ -32,6 +32,23 @@ class VBasisBF:
     normalization: float = 1.0
     _device_cache: dict = field(default_factory=dict, compare=False, repr=False)
 
+    def restrict(self, channel_indices) -> "VBasisBF":
+        """The rows of the K subsets that make up `channel_indices` (whole K
+        subsets only), with channel numbers local to that list."""
+        local = {int(g): i for i, g in enumerate(channel_indices)}
+        chan, Br, Bi = {}, {}, {} if self.B_imag is not None else None
+        for K, idx in self.channel_indices.items():
+            inside = [int(g) in local for g in idx]
+            if not any(inside):
+                continue
+            if not all(inside):
+                raise fail(ValueError, f"block channels must contain every channel of helicity K={K} or none")
+            chan[K] = [local[int(g)] for g in idx]
+            Br[K] = self.B_real[K]
+            if Bi is not None:
+                Bi[K] = self.B_imag[K]
+        return VBasisBF(len(local), self.grid_shape, chan, Br, Bi, self.normalization)
+
     def device_arrays(self, device: torch.device):
         """Concatenated (rows grouped by K) device copies, cached per device."""
         key = (device.type, device.index)

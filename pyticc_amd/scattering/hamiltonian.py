@@ -8,7 +8,7 @@ into W_base = U/R^2 + 2 mu V + 2 mu diag(E_int).
 
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from math import prod
 
 import numpy as np
@@ -53,6 +53,15 @@ class ScattHamiltonian:
             c = (_device.to_device(self.U, device), _device.to_device(self.E_int, device))
             self._cache[key] = c
         return c
+
+    def block(self, channel_indices) -> "ScattHamiltonian":
+        """The Hamiltonian projected on a CS/NNCC block (whole K subsets): its
+        channels, their weighted basis rows, U restricted to them (Coriolis
+        couplings only between K values inside the block)."""
+        indices = tuple(int(i) for i in channel_indices)
+        basis = replace(self.basis, channels=tuple(self.basis[i] for i in indices))
+        return ScattHamiltonian(basis, self.reduced_mass, self.vbasis.restrict(indices), self.potential_grid, self.approx,
+                                self.K_delta)
 
     def W_points(self, radial_points, device: torch.device, out: torch.Tensor | None = None) -> torch.Tensor:
         """Energy-independent W_base at the given grid points, (n_R, N, N) on the device."""

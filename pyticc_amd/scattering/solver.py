@@ -1,8 +1,10 @@
-"""ticc.solve for the exact coupled-channel path (src/pyticc/scattering/solver.py:57-196).
+"""ticc.solve for the inelastic coupled-channel path (src/pyticc/scattering/solver.py:57-217).
 
-Everything after the one-time host setup (channel list, weighted basis rows,
-U, BF->SF eigenvectors) runs on the device: contraction, LDMD sweep and the
-fused BF->SF + matching kernel.
+Exact CC propagates the whole (J, parity) block; CS and NNCC propagate and match
+each K block independently (solver.py:154-177, propagation/runner.py:326-374,
+match/finalize.py:168-225).  Everything after the one-time host setup (channel
+list, weighted basis rows, U, BF->SF eigenvectors) runs on the device:
+contraction, LDMD sweep and the fused BF->SF + matching kernel.
 """
 
 from __future__ import annotations
@@ -19,7 +21,8 @@ from pyticc_amd.match.asymptotic import get_Bmat_BF_to_SF
 from pyticc_amd.match.smatrix import get_Smat_device, unpack_smat
 from pyticc_amd.propagation.device import resolve_device
 from pyticc_amd.propagation.runner import propagate
-from pyticc_amd.result import ScatteringResult, Timing
+from pyticc_amd.basis.kblock import build_cs_blocks, build_nncc_blocks
+from pyticc_amd.result import CoupledStatesResult, KBlockResult, ScatteringResult, Timing
 from pyticc_amd.scattering.hamiltonian import ScattHamiltonian, build_hamiltonian
 from pyticc_amd.scattering.potential import PotentialGrid
 from pyticc_amd.system import Approx, ScattSystem
@@ -39,11 +42,11 @@ def solve(problem, Etot, potential_grid, propagation) -> ScatteringResult:
         sectors = tuple(potential_grid) if not isinstance(potential_grid, PotentialGrid) else potential_grid.sectors
     else:
         raise fail(TypeError, f"solve expects a ScattSystem or ScattHamiltonian, got {type(problem).__name__}")
-    if hamiltonian.approx is not Approx.EXACT:
-        raise fail(NotImplementedError, "CS/NNCC block propagation is not supported by the MI355X backend")
     energies = get_Etot(Etot)
     basis = hamiltonian.basis
     wall0, cpu0 = perf_counter(), process_time()
+    if hamiltonian.approx is not Approx.EXACT:
+        return _solve_blocks(hamiltonian, energies, sectors, propagation, wall0, cpu0)
     logger.info(f"Solving Jtot={basis.Jtot}, system_parity={basis.system_parity:+d}, approx=exact, channels={basis.n_channel}, energies={energies.size}")
     B, L = get_Bmat_BF_to_SF(basis)
     Y, S_re, S_im, n_open = solve_device(hamiltonian, energies, sectors, propagation, B, L)
@@ -62,3 +65,34 @@ def solve_device(hamiltonian, energies, sectors, propagation, B, L):
     dev = resolve_device(propagation.device).torch_device
     S_re, S_im, n_open = get_Smat_device(Y, B, sectors[-1].radial_end, energies, hamiltonian.reduced_mass, hamiltonian.basis.E_int, L, dev)
     return Y, S_re, S_im, n_open
+
+
+def build_k_blocks(hamiltonian) -> tuple:
+    """CS or NNCC propagation blocks of one Hamiltonian (solver.py:203-214)."""
+    if hamiltonian.approx is Approx.CS:
+        return build_cs_blocks(hamiltonian.basis)
+    if hamiltonian.approx is Approx.NNCC:
+        return build_nncc_blocks(hamiltonian.basis, hamiltonian.K_delta)
+    raise fail(ValueError, f"Coupled-states blocks require approx='cs' or 'nncc', but got {hamiltonian.approx.value!r}")
+
+
+def _solve_blocks(hamiltonian, energies, sectors, propagation, wall0, cpu0) -> CoupledStatesResult:
+    """Propagate and match every K block on the device (finalize.py:168-225 per block)."""
+    basis = hamiltonian.basis
+    blocks = build_k_blocks(hamiltonian)
+    logger.info(f"Solving Jtot={basis.Jtot}, system_parity={basis.system_parity:+d}, approx={hamiltonian.approx.value}, "
+                f"channels={basis.n_channel}, blocks={len(blocks)}, energies={energies.size}")
+    results = []
+    for block in blocks:
+        sub = hamiltonian.block(block.channel_indices)
+        B, L = get_Bmat_BF_to_SF(sub.basis)
+        Y, S_re, S_im, n_open = solve_device(sub, energies, sectors, propagation, B, L)
+        Smat = unpack_smat(S_re, S_im, n_open)
+        indices = np.asarray(block.channel_indices, dtype=np.int64)
+        E_int = sub.basis.E_int
+        open_idx = tuple(indices[np.flatnonzero(E_int < E)] for E in energies)
+        results.append(KBlockResult(block=block, open_channel_indices=open_idx, Y_BF=_device.to_host(Y), Bmat=B, L=L,
+                                    Smat_asymptotic=Smat))
+    timing = Timing(wall_seconds=perf_counter() - wall0, cpu_seconds=process_time() - cpu0)
+    logger.info(f"Solver complete: {timing}")
+    return CoupledStatesResult(basis=basis, Etot=energies, approx=hamiltonian.approx, blocks=tuple(results), timing=timing)

@@ -20,7 +20,7 @@ from scipy.special import eval_legendre, gammaln, lpmv
 from pyticc_amd.basis.monomer import AtomSpec, rigid_rotor
 from pyticc_amd.constants import AMU2AU, CM2AU
 from pyticc_amd.pes import PESWrapper
-from pyticc_amd.system import build_ScattSystem
+from pyticc_amd.system import Approx, build_ScattSystem
 
 
 def tang_toennies_f6(x: np.ndarray) -> np.ndarray:
@@ -114,10 +114,11 @@ class Workload:
 
 def atom_rotor(seed: int = 1, *, jmax: int = 10, Jtot: int = 10, parity: int = 1, n_energy: int = 20,
                boundaries=(3.0, 30.0), half_step: float = 0.01, n_theta: int = 20, B_rot_cm: float = 10.0,
-               mu_amu: float = 10.0, e_range_cm=(50.0, 500.0)) -> Workload:
+               mu_amu: float = 10.0, e_range_cm=(50.0, 500.0), approx: Approx = Approx.EXACT, K_delta: int = 1) -> Workload:
     """Atom + rigid rotor workload (default: N = 66, 20 energies, 1350 sectors)."""
     system = build_ScattSystem(AtomSpec(), rigid_rotor(B_rot_cm * CM2AU, jmax), Jtot=Jtot, system_parity=parity,
-                               potential=AtomRotorPES.seeded(seed).wrapper(), reduced_mass=mu_amu * AMU2AU)
+                               approx=approx, K_delta=K_delta, potential=AtomRotorPES.seeded(seed).wrapper(),
+                               reduced_mass=mu_amu * AMU2AU)
     E = np.linspace(*e_range_cm, n_energy) * CM2AU
     return Workload("atom_rotor", system, tuple(boundaries), (half_step,), {"n_theta": n_theta}, E)
 
@@ -125,11 +126,12 @@ def atom_rotor(seed: int = 1, *, jmax: int = 10, Jtot: int = 10, parity: int = 1
 def rotor_rotor(seed: int = 2, *, jmax_X: int = 4, jmax_Y: int = 9, Jtot: int = 4, parity: int = -1,
                 n_energy: int = 64, boundaries=(4.0, 40.0), half_step: float = 0.009, n_theta_X: int = 15,
                 n_theta_Y: int = 15, n_phi: int = 12, B_X_cm: float = 10.0, B_Y_cm: float = 7.3,
-                mu_amu: float = 10.0, e_range_cm=(10.87, 1000.87)) -> Workload:
+                mu_amu: float = 10.0, e_range_cm=(10.87, 1000.87), approx: Approx = Approx.EXACT,
+                K_delta: int = 1) -> Workload:
     """Two rigid rotors (default: N = 800, 64 energies, 2000 sectors, 15 x 15 x 12 angular grid)."""
     system = build_ScattSystem(rigid_rotor(B_X_cm * CM2AU, jmax_X), rigid_rotor(B_Y_cm * CM2AU, jmax_Y), Jtot=Jtot,
-                               system_parity=parity, potential=RotorRotorPES.seeded(seed).wrapper(),
-                               reduced_mass=mu_amu * AMU2AU)
+                               system_parity=parity, approx=approx, K_delta=K_delta,
+                               potential=RotorRotorPES.seeded(seed).wrapper(), reduced_mass=mu_amu * AMU2AU)
     E = np.linspace(*e_range_cm, n_energy) * CM2AU
     quad = {"n_theta_X": n_theta_X, "n_theta_Y": n_theta_Y, "n_phi": n_phi}
     return Workload("rotor_rotor", system, tuple(boundaries), (half_step,), quad, E)

@@ -12,25 +12,28 @@ from pyticc_amd.scattering.hamiltonian import build_hamiltonian
 
 def oracle_problem(workload):
     """Prepare the PES grid and the oracle Problem for a synthetic workload."""
-    sysm = workload.system
-    grid = prepare_potential(sysm, workload.boundaries, workload.half_steps, **workload.quadrature)
-    ham = build_hamiltonian(sysm, grid)
+    grid = prepare_potential(workload.system, workload.boundaries, workload.half_steps, **workload.quadrature)
+    ham = build_hamiltonian(workload.system, grid)
+    return grid, ham, hamiltonian_problem(ham, grid)
+
+
+def hamiltonian_problem(ham, grid):
+    """Oracle Problem of a (possibly block-restricted) Hamiltonian on a prepared grid."""
     vb = ham.vbasis
     blocks = [(np.asarray(vb.channel_indices[K]), vb.B_real[K], None if vb.B_imag is None else vb.B_imag[K]) for K in sorted(vb.channel_indices)]
-    basis = sysm.basis
-    prob = Problem(
+    basis = ham.basis
+    return Problem(
         blocks=blocks,
         norm=vb.normalization,
         U=get_Umat_BF(basis),
         E_int=basis.E_int,
-        mu=sysm.reduced_mass,
+        mu=ham.reduced_mass,
         ladder=ladder_labels(basis),
         j_couple=np.asarray([c.j_couple for c in basis]),
         Jtot=basis.Jtot,
         sectors=[(s.radial_start, s.radial_end) for s in grid.sectors],
         V=grid.values.reshape(grid.radial_points.size, -1),
     )
-    return grid, ham, prob
 
 
 def random_W_batch(rng, n_sector, N, scale=1.0, diag_shift=0.0):
