@@ -132,9 +132,11 @@ __device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long
   constexpr int NW = THREADS / 64;
   __shared__ double s_vals[2][NW][RPT][NB];
   __shared__ double s_inv[NB];
-  __shared__ double s_v[2][NW];
+  __shared__ unsigned long long s_key[2][NW];
   __shared__ int s_pos[2][NW];
   __shared__ int s_row[2][NW];
+  __shared__ double s_pinv[2][NW];
+  __shared__ double s_pnext[2][NW];
 
   const int t = threadIdx.x;
   const int lane = t & 63;
@@ -162,65 +164,113 @@ __device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long
     }
   }
 
+  // The column loop runs PU columns per iteration on a rotated register
+  // window: v[q][l] holds panel column (j0 + l) % NB, so every iteration's
+  // code is the same and the kernel stays small enough for the instruction
+  // cache (a fully unrolled 32-column loop does not fit and runs from L2).
+  constexpr int PU = 4;
+  for (int j0 = 0; j0 < NB; j0 += PU) {
 #pragma unroll
-  for (int j = 0; j < NB; ++j) {
+  for (int u = 0; u < PU; ++u) {
+    const int j = j0 + u;
     const int c = k0 + j;
-    const int buf = j & 1;
-    // candidate: largest |a| among rows at positions >= c (NaN ranks as +inf so
-    // a column always has a pivot and the NaN propagates to the result)
+    const int buf = u & 1;
+    // This thread's candidate: largest |a| among its rows at positions >= c,
+    // first position on ties (NaN ranks as +inf so a column always has a pivot
+    // and the NaN propagates to the result).
     double cv = -1.0;
     int cp = 0x7fffffff;
     int cq = -1;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-      if (pos[q] >= c) {
-        double a = fabs(v[q][j]);
-        if (a != a) a = __builtin_inf();
-        if (better(a, pos[q], cv, cp)) {
-          cv = a;
-          cp = pos[q];
-          cq = q;
+      double a = fabs(v[q][u]);
+      a = (a != a) ? __builtin_inf() : a;
+      const bool take = (pos[q] >= c) && better(a, pos[q], cv, cp);
+      cv = take ? a : cv;
+      cp = take ? pos[q] : cp;
+      cq = take ? q : cq;
+    }
+    // Exact ordering key of the candidate: the bits of |a| plus one (monotone in
+    // |a|), 0 for no candidate.  The wave maximum of its top 32 bits takes one
+    // DPP max per step; only when several lanes share that maximum is the exact
+    // (|a|, position) argmax run among the wave.
+    const unsigned long long key = (cq >= 0) ? (unsigned long long)__double_as_longlong(cv) + 1ull : 0ull;
+    const unsigned khi = (unsigned)(key >> 32);
+    unsigned m = khi;
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x128, 0xF, 0xF, false));  // row_ror:8
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x124, 0xF, 0xF, false));  // row_ror:4
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x122, 0xF, 0xF, false));  // row_ror:2
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x121, 0xF, 0xF, false));  // row_ror:1
+    const unsigned M = max(max((unsigned)__builtin_amdgcn_readlane((int)m, 0), (unsigned)__builtin_amdgcn_readlane((int)m, 16)),
+                           max((unsigned)__builtin_amdgcn_readlane((int)m, 32), (unsigned)__builtin_amdgcn_readlane((int)m, 48)));
+    int wl = -1;
+    if (M != 0u) {
+      const unsigned long long tied = __ballot(khi == M);
+      if (__popcll(tied) == 1) {
+        wl = __ffsll((long long)tied) - 1;
+      } else {
+        double wv = (khi == M) ? cv : -1.0;
+        int wp = (khi == M) ? cp : 0x7fffffff;
+        wave_argmax_dpp(wv, wp);
+        wl = __ffsll((long long)__ballot(khi == M && cv == wv && cp == wp)) - 1;
+      }
+    }
+    // The wave's winning lane publishes its row (each of a thread's rows has its
+    // own LDS slot, so the write needs no register select), its key, position,
+    // row, the reciprocal of its pivot value and its value in the next column.
+    if (lane == wl) {
+      double pv = 0.0, pn = 0.0;
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        if (cq == q) {
+#pragma unroll
+          for (int l = 0; l < NB; ++l) s_vals[buf][wave][q][l] = v[q][l];
+          pv = v[q][u];
+          pn = v[q][u + 1];
         }
       }
-    }
-    double wv = cv;
-    int wp = cp;
-    wave_argmax_dpp(wv, wp);
-    // the lane holding the wave's winning row publishes it; each of a thread's
-    // rows has its own LDS slot so the write needs no register select
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      if (cq == q && cp == wp) {
-#pragma unroll
-        for (int l = 0; l < NB; ++l) s_vals[buf][wave][q][l] = v[q][l];
-        s_row[buf][wave] = t + THREADS * q;
-      }
-    }
-    if (lane == 0) {
-      s_v[buf][wave] = wv;
-      s_pos[buf][wave] = wp;
+      s_key[buf][wave] = key;
+      s_pos[buf][wave] = cp;
+      s_row[buf][wave] = t + THREADS * cq;
+      s_pinv[buf][wave] = 1.0 / pv;
+      s_pnext[buf][wave] = pn;
+    } else if (wl < 0 && lane == 0) {
+      s_key[buf][wave] = 0ull;
+      s_pos[buf][wave] = 0x7fffffff;
     }
     __syncthreads();
-    double best_v = s_v[buf][0];
-    int bp = s_pos[buf][0];
-    int bw = 0;
+    // Global winner: larger key, smaller position on ties (a pairwise tree).
+    unsigned long long kw[NW];
+    int pw[NW], ww[NW];
 #pragma unroll
-    for (int w = 1; w < NW; ++w) {
-      if (better(s_v[buf][w], s_pos[buf][w], best_v, bp)) {
-        best_v = s_v[buf][w];
-        bp = s_pos[buf][w];
-        bw = w;
+    for (int w = 0; w < NW; ++w) {
+      kw[w] = s_key[buf][w];
+      pw[w] = s_pos[buf][w];
+      ww[w] = w;
+    }
+#pragma unroll
+    for (int h = 1; h < NW; h <<= 1) {
+#pragma unroll
+      for (int w = 0; w + h < NW; w += 2 * h) {
+        const bool tb = (kw[w + h] > kw[w]) || (kw[w + h] == kw[w] && pw[w + h] < pw[w]);
+        kw[w] = tb ? kw[w + h] : kw[w];
+        pw[w] = tb ? pw[w + h] : pw[w];
+        ww[w] = tb ? ww[w + h] : ww[w];
       }
     }
+    const int bw = ww[0];
+    const int bp = pw[0];
     const int pr = s_row[buf][bw];
+    const double inv = s_pinv[buf][bw];
+    const double pn = s_pnext[buf][bw];
     const double* prow = s_vals[buf][bw][pr / THREADS];
-    const double inv = 1.0 / prow[j];
     // Gauss-Jordan step with the pivot row's scaling deferred: every other row
     // r takes a_rl -= (a_rj / p) p_l, a_rj = -a_rj / p; the pivot row keeps its
     // values with a_rj = 1.  A row that was a pivot row stays p times its true
     // values through the later columns of the panel (the update is linear in
     // the row) and is multiplied by 1/p (kept in s_inv) once at the end.  The
     // pivot row's own update is a no-op through g = 0, so no lane branches.
+    // The next column is updated first: the next pivot search needs only it.
     if (t == 0) s_inv[j] = inv;
     double g[RPT];
 #pragma unroll
@@ -228,16 +278,33 @@ __device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long
       const bool is_piv = (t + THREADS * q == pr);
       if (pos[q] == c) pos[q] = bp;
       pos[q] = is_piv ? c : pos[q];
-      g[q] = is_piv ? 0.0 : v[q][j] * inv;
+      g[q] = is_piv ? 0.0 : v[q][u] * inv;
     }
+    // (u + 1 is the next column, or for the last column another column of
+    // the window; pn is the pivot row's value there either way)
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) v[q][u + 1] = fma(-g[q], pn, v[q][u + 1]);
 #pragma unroll
     for (int l = 0; l < NB; ++l) {
+      if (l == u || l == u + 1) continue;
       const double pl = prow[l];
 #pragma unroll
       for (int q = 0; q < RPT; ++q) v[q][l] = fma(-g[q], pl, v[q][l]);
     }
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) v[q][j] = (t + THREADS * q == pr) ? 1.0 : -g[q];
+    for (int q = 0; q < RPT; ++q) v[q][u] = (t + THREADS * q == pr) ? 1.0 : -g[q];
+    }
+    // rotate the window by PU columns
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      double tmp[PU];
+#pragma unroll
+      for (int l = 0; l < PU; ++l) tmp[l] = v[q][l];
+#pragma unroll
+      for (int l = 0; l + PU < NB; ++l) v[q][l] = v[q][l + PU];
+#pragma unroll
+      for (int l = 0; l < PU; ++l) v[q][NB - PU + l] = tmp[l];
+    }
   }
 
   // Deferred pivot-row scaling of this panel's pivot rows.
@@ -301,12 +368,29 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
                               prev_piv, prev_k0, src2, brow, blockIdx.x);
 }
 
+// Raw buffer loads: an offset past the end of the buffer reads 0 without a
+// memory access, so masked rows need no branch (a branch around a load makes
+// the compiler wait for that load at the join).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+constexpr int kOutOfRange = 0x7ff00000;
+__device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ double2 buffer_load_f64x2(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
 // Small panel GEMM (FP64 MFMA) of the paired block step, for one matrix b:
 //   Out[i][0:NB] = mask(i) C[gC(i)][colC : colC+NB] + A[i][0:NB] . B[gB(r)][colB : colB+NB],
 //   mask(i) = 0 for i in [mlo, mlo+NB), 1 otherwise.
 // Out is row-major (Np x NB) or, with `ostride` > 0, transposed (NB rows of
-// length Np, matrix stride ostride).  The workgroup's waves take 16-row blocks
-// in turn; sB stages the NB gathered B rows.
+// length Np, matrix stride ostride).  Wave w takes the 16-row blocks
+// w, w + THREADS/64, ...; the row-source indices and then the C and A
+// operands of GCH blocks are loaded together, so the gathers' latency is paid
+// twice per GCH blocks instead of twice per block.  sB stages the NB gathered
+// B rows.
 template <int THREADS>
 __device__ __forceinline__ void panel_gemm_body(const double* __restrict__ A, const double* __restrict__ C,
                                                 long long cb, int ldC, int colC, const int* __restrict__ gC,
@@ -314,52 +398,82 @@ __device__ __forceinline__ void panel_gemm_body(const double* __restrict__ A, co
                                                 const int* __restrict__ gB, int mlo, int Np,
                                                 double* __restrict__ Out, long long ostride, int b,
                                                 double (*sB)[NB + 1]) {
+  constexpr int RB = THREADS / 4;                 // rows per round of all waves
+  constexpr int MAXIT = (MAX_NP + RB - 1) / RB;   // rounds at the largest Np
+  constexpr int GCH = 4;                          // rounds whose operands are in flight together
+  static_assert(MAXIT % GCH == 0, "round count must be a multiple of the chunk");
   const int t = threadIdx.x;
   const int wave = t >> 6, lane = t & 63, lr = lane & 15, lq = lane >> 4;
-  const double* Ab = A + (size_t)b * Np * NB;
   const double* Cb = C + (size_t)b * cb;
+  const auto rC = make_rsrc(Cb, (unsigned)(sizeof(double) * Np * ldC));
+  const auto rA = make_rsrc(A + (size_t)b * Np * NB, (unsigned)(sizeof(double) * Np * NB));
+  const auto rG = make_rsrc(gC + (size_t)b * Np, (unsigned)(sizeof(int) * Np));
   const double* Bb = Bm + (size_t)b * bb;
-  const int* gCb = gC + (size_t)b * Np;
   const int* gBb = gB + b * NB;
   for (int e = t; e < NB * NB; e += THREADS) {
     const int r = e / NB, cc = e % NB;
     sB[r][cc] = Bb[(size_t)gBb[r] * ldB + colB + cc];
   }
   __syncthreads();
-  for (int row0 = wave * 16; row0 < Np; row0 += THREADS / 4) {
-    double4_t acc[2];
+  for (int c0 = 0; c0 < MAXIT; c0 += GCH) {
+    if (c0 * RB >= Np) break;
+    // source row of each accumulator row (-1: masked or past the end)
+    int src[GCH][4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = row0 + mfma_drow(lq, r);
-      const bool m = (row >= mlo) && (row < mlo + NB);
-      const double* cr = Cb + (size_t)gCb[row] * ldC + colC + lr;
+    for (int u = 0; u < GCH; ++u)
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj) {
-        const double cv = cr[16 * bj];
-        acc[bj][r] = m ? 0.0 : cv;
+      for (int r = 0; r < 4; ++r) {
+        const int row = (c0 + u) * RB + wave * 16 + mfma_drow(lq, r);
+        const bool live = row < Np && !(row >= mlo && row < mlo + NB);
+        const int g = (int)__builtin_amdgcn_raw_buffer_load_b32(rG, live ? 4 * row : kOutOfRange, 0, 0);
+        src[u][r] = live ? g : -1;
+      }
+    double4_t acc[GCH][2];
+    double a[GCH][NB / 4];
+#pragma unroll
+    for (int u = 0; u < GCH; ++u) {
+      const int row0 = (c0 + u) * RB + wave * 16;
+#pragma unroll
+      for (int kk = 0; kk < NB / 4; ++kk)
+        a[u][kk] = buffer_load_f64(rA, row0 + lr < Np ? 8 * ((row0 + lr) * NB + 4 * kk + lq) : kOutOfRange);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int s = src[u][r];
+        const int base = s >= 0 ? 8 * (s * ldC + colC + lr) : kOutOfRange;
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) acc[u][bj][r] = buffer_load_f64(rC, base + 128 * bj);
       }
     }
 #pragma unroll
-    for (int kk = 0; kk < NB / 4; ++kk) {
-      const int kq = 4 * kk + lq;
-      const double a = Ab[(size_t)(row0 + lr) * NB + kq];
+    for (int u = 0; u < GCH; ++u) {
+      const int row0 = (c0 + u) * RB + wave * 16;
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-        acc[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sB[kq][16 * bj + lr], acc[bj], 0, 0, 0);
-    }
-    if (ostride > 0) {
-      double* Ob = Out + (size_t)b * ostride;
+      for (int kk = 0; kk < NB / 4; ++kk) {
+        const int kq = 4 * kk + lq;
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+        for (int bj = 0; bj < 2; ++bj)
+          acc[u][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][kk], sB[kq][16 * bj + lr], acc[u][bj], 0, 0, 0);
+      }
+      if (ostride > 0) {
+        double* Ob = Out + (size_t)b * ostride;
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) Ob[(size_t)(16 * bj + lr) * Np + row0 + mfma_drow(lq, r)] = acc[bj][r];
-    } else {
-      double* Ob = Out + (size_t)b * Np * NB;
+        for (int r = 0; r < 4; ++r) {
+          const int row = row0 + mfma_drow(lq, r);
+          if (row < Np) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = row0 + mfma_drow(lq, r);
+            for (int bj = 0; bj < 2; ++bj) Ob[(size_t)(16 * bj + lr) * Np + row] = acc[u][bj][r];
+          }
+        }
+      } else {
+        double* Ob = Out + (size_t)b * Np * NB;
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) Ob[(size_t)row * NB + 16 * bj + lr] = acc[bj][r];
+        for (int r = 0; r < 4; ++r) {
+          const int row = row0 + mfma_drow(lq, r);
+          if (row < Np) {
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) Ob[(size_t)row * NB + 16 * bj + lr] = acc[u][bj][r];
+          }
+        }
       }
     }
   }
@@ -405,20 +519,6 @@ __device__ __forceinline__ void gj_pair_chain_body(const PairJob& P, int b) {
 template <int RPT>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_pair_chain(PairJob P) {
   gj_pair_chain_body<RPT>(P, blockIdx.x);
-}
-
-// Raw buffer loads: an offset past the end of the buffer reads 0 without a
-// memory access, so masked rows need no branch (a branch around a load makes
-// the compiler wait for that load at the join).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
-}
-constexpr int kOutOfRange = 0x7ff00000;
-__device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t r, int off) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-__device__ __forceinline__ double2 buffer_load_f64x2(__amdgpu_buffer_rsrc_t r, int off) {
-  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
 
 // Rank-KB update (KB = NB for one step, 2 NB for a pair of steps):
@@ -481,9 +581,12 @@ __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, dou
     if (!rows_ok || j0 >= Np) return 2;
     return (j0 >= k0 && j0 < k0 + KB) ? 1 : 0;
   };
-  auto load_c = [&](int J, double4_t(&c)[2][2]) {
+  // live = false: every load takes the out-of-range offset (reads 0, no
+  // memory access).  The prefetches are issued on every path so the compiler's
+  // wait counts before the current tile's MFMAs need not cover them.
+  auto load_c = [&](int J, bool live, double4_t(&c)[2][2]) {
     const int j0 = J * 64 + jw;
-    const bool upd = kind(J) == 0;
+    const bool upd = live && kind(J) == 0;
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -495,8 +598,8 @@ __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, dou
       }
   };
   double2 bst[KB / RP];
-  auto load_b = [&](int J) {
-    const bool in = (J * 64 + c2) < Np;
+  auto load_b = [&](int J, bool live) {
+    const bool in = live && (J * 64 + c2) < Np;
 #pragma unroll
     for (int h = 0; h < KB / RP; ++h)
       bst[h] = buffer_load_f64x2(rX, in ? 8 * (bro[h] * Np + J * 64 + c2) : kOutOfRange);
@@ -557,27 +660,25 @@ __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, dou
   };
 
   double4_t accA[2][2], accB[2][2];
-  load_b(jt0);
-  load_c(jt0, accA);
+  load_b(jt0, true);
+  load_c(jt0, true, accA);
   store_b(0);
   __syncthreads();
   for (int J = jt0;; J += 2) {
     // even tile: accumulators A, LDS buffer 0; prefetch into B / buffer 1
-    if (J + 1 < jt1) {
-      load_b(J + 1);
-      load_c(J + 1, accB);
-    }
+    const bool more1 = J + 1 < jt1;
+    load_b(J + 1, more1);
+    load_c(J + 1, more1, accB);
     compute_store(J, 0, accA);
-    if (J + 1 >= jt1) break;
+    if (!more1) break;
     store_b(1);
     __syncthreads();
     // odd tile: accumulators B, LDS buffer 1; prefetch into A / buffer 0
-    if (J + 2 < jt1) {
-      load_b(J + 2);
-      load_c(J + 2, accA);
-    }
+    const bool more2 = J + 2 < jt1;
+    load_b(J + 2, more2);
+    load_c(J + 2, more2, accA);
     compute_store(J + 1, 1, accB);
-    if (J + 2 >= jt1) break;
+    if (!more2) break;
     store_b(0);
     __syncthreads();
   }
