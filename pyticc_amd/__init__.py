@@ -36,6 +36,7 @@ from pyticc_amd.pes import PESWrapper
 from pyticc_amd.propagation import Propagation
 from pyticc_amd.result import CoupledStatesResult, KBlockResult, ScatteringResult, Timing
 from pyticc_amd.scattering import PotentialGrid, ScattHamiltonian, prepare_potential, solve
+from pyticc_amd import report
 from pyticc_amd.system import Approx, ChannelSpec, MolInnerState, ScatteringType, ScattSystem, build_ScattSystem, element_mass_au, reduced_mass
 
 __all__ = [
@@ -44,5 +45,5 @@ __all__ = [
     "DiatomSpec", "EV2AU", "GHZ2AU", "HZ2AU", "K2AU", "KBlock", "KBlockResult", "KHZ2AU", "MHZ2AU", "MolInnerState", "PESWrapper", "PotentialGrid", "Propagation",
     "RovibBasis", "ScatteringResult", "ScatteringType", "ScattHamiltonian", "ScattSystem", "Timing",
     "build_ChannelBasis", "build_ScattSystem", "build_cs_blocks", "build_nncc_blocks", "element_mass_au", "get_Etot", "prepare_potential", "reduced_mass",
-    "rigid_rotor", "solve",
+    "report", "rigid_rotor", "solve",
 ]

@@ -47,6 +47,11 @@ def test_solve_config1_matches_fixture(gpu_device):
         worst = max(worst, float(np.max(np.abs(res.Smat[e] - ref))))
     assert worst <= TOL_S, worst
     assert max(omatch.unitarity_defect(s) for s in res.Smat if s.size) < 1e-10
+    # the S-matrix text table carries the same elements, S[out, in], incoming outermost
+    e = int(d["energy_index"][0])
+    lines = ticc.report.smatrix(res, energy_indices=e).splitlines()[2:]
+    vals = np.array([complex(float(x.split()[-2]), float(x.split()[-1])) for x in lines])
+    np.testing.assert_allclose(vals, res.Smat[e].T.ravel(), rtol=1e-15, atol=1e-16)
 
 
 def test_solve_config2_full_size_matches_fixture(gpu_device):
