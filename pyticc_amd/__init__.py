@@ -33,6 +33,7 @@ from pyticc_amd.constants import (
 )
 from pyticc_amd.energy import get_Etot
 from pyticc_amd.pes import PESWrapper
+from pyticc_amd.pes.native import load_native_pes
 from pyticc_amd.propagation import Propagation
 from pyticc_amd.result import CoupledStatesResult, KBlockResult, ScatteringResult, Timing
 from pyticc_amd.scattering import PotentialGrid, ScattHamiltonian, prepare_potential, solve
@@ -44,6 +45,6 @@ __all__ = [
     "Approx", "AtomSpec", "CM2AU", "Channel", "ChannelBasis", "ChannelSpec", "CoupledStatesResult", "DiatomBasis",
     "DiatomSpec", "EV2AU", "GHZ2AU", "HZ2AU", "K2AU", "KBlock", "KBlockResult", "KHZ2AU", "MHZ2AU", "MolInnerState", "PESWrapper", "PotentialGrid", "Propagation",
     "RovibBasis", "ScatteringResult", "ScatteringType", "ScattHamiltonian", "ScattSystem", "Timing",
-    "build_ChannelBasis", "build_ScattSystem", "build_cs_blocks", "build_nncc_blocks", "element_mass_au", "get_Etot", "prepare_potential", "reduced_mass",
+    "build_ChannelBasis", "build_ScattSystem", "build_cs_blocks", "build_nncc_blocks", "element_mass_au", "get_Etot", "load_native_pes", "prepare_potential", "reduced_mass",
     "report", "rigid_rotor", "solve",
 ]

@@ -18,6 +18,7 @@ class PESWrapper:
     monomer_X: Callable | None = None
     monomer_Y: Callable | None = None
     interaction_many: Callable | None = None
+    _interaction_many_processes: Callable | None = field(default=None, repr=False, compare=False)
     _close: Callable | None = field(default=None, repr=False, compare=False)
 
     def close(self) -> None:
@@ -36,6 +37,8 @@ def _evaluate(pes: PESWrapper, R, coordinates: np.ndarray, grid_shape, *, proces
     elif Rs.ndim == 1:
         if Rs.size == 0:
             values = np.empty((0, coordinates.shape[1]))
+        elif pes._interaction_many_processes is not None:
+            values = np.asarray(pes._interaction_many_processes(Rs, coordinates, processes), dtype=np.float64)
         elif pes.interaction_many is None:
             values = np.stack([pes.interaction(float(r), coordinates) for r in Rs])
         else:
