@@ -113,6 +113,32 @@ int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, dou
                         long long w_sector_stride, double* Y, void* stream);
 
 /*
+ * Capture boundary condition (Propagation(mode="capture")): complex symmetric
+ * log derivatives Z carried in the real form [[Re Z, -Im Z], [Im Z, Re Z]]
+ * (2N x 2N), on which sums, products, inverses and real diagonal shifts act as
+ * on Z, so the sweep is the real sweep of size 2N.
+ *
+ * Replaces initialize_logD_capture (src/pyticc/propagation/logd.py:72-91), as
+ * called at src/pyticc/propagation/runner.py:284-291: Z_e = diag(z),
+ * z_i = sqrt(d) for d = W0_ii - 2 mu E_e >= 0, -i sqrt(-d) otherwise.
+ *   W0 device (N, N); Y device (n_energy, 2N, 2N) output (real form).
+ */
+int ticc_ldmd_init_capture(int n_energy, int n_channel, const double* W0, const double* energies,
+                           double reduced_mass, double* Y, void* stream);
+
+/*
+ * LDMD sweep of complex log derivatives in real form (the capture mode of
+ * propagate_logD, src/pyticc/propagation/logd.py:288-330, complex128 there).
+ * Same arguments as ticc_ldmd_propagate with n_channel = N, but W_start,
+ * W_mid, W_end point to the real form of W_base, block-diag(W, W) (2N x 2N),
+ * and Y holds the real forms of the energies' Z (n_energy, 2N, 2N).  The
+ * end-of-sector symmetrisation is Z -> (Z + Z^T)/2 in real form.
+ */
+int ticc_ldmd_propagate_capture(int n_energy, int n_channel, const double* energies, double reduced_mass,
+                                int n_sector, const double* half_steps, const double* W_start, const double* W_mid,
+                                const double* W_end, long long w_sector_stride, double* Y, void* stream);
+
+/*
  * Asymptotic matching, kernel C: Y_SF = B^T Y B, then per energy the
  * closed-channel elimination and K -> S.
  * Replaces transform_logD_BF_to_SF (src/pyticc/match/asymptotic.py:82-110) and

@@ -40,6 +40,11 @@ _SIGNATURES = {
         [_c_int, _c_int, _dp, _c_d, _c_int, _dp, _vp, _vp, _vp, _c_ll, _vp, _vp],
         _c_int,
     ),
+    "ticc_ldmd_init_capture": ([_c_int, _c_int, _vp, _dp, _c_d, _vp, _vp], _c_int),
+    "ticc_ldmd_propagate_capture": (
+        [_c_int, _c_int, _dp, _c_d, _c_int, _dp, _vp, _vp, _vp, _c_ll, _vp, _vp],
+        _c_int,
+    ),
     "ticc_get_smat": ([_c_int, _c_int, _vp, _vp, _c_d, _dp, _c_d, _dp, _dp, _vp, _vp, _ip, _vp], _c_int),
     "ticc_bessel_eval": ([_c_int, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "ticc_invert_batched": ([_c_int, _c_int, _vp, _vp], _c_int),

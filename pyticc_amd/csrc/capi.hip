@@ -10,8 +10,9 @@ int contract_w(int N, int n_ang, int n_block, const int* block_offsets, const in
                const double* Bi, double norm, int nR, const double* R, const double* V, const double* U,
                const double* Eint, double mu, double* W, hipStream_t stream);
 int ldmd_init(int nE, int N, const double* W0, const double* E_host, double mu, double* Y, hipStream_t stream);
+int ldmd_init_capture(int nE, int N, const double* W0, const double* E_host, double mu, double* Y, hipStream_t stream);
 int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const double* hs_host, const double* Ws0,
-                   const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream);
+                   const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream, int half);
 int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, const double* E_host, double mu,
              const double* Eint_host, const double* L_host, double* S_re, double* S_im, int* n_open,
              hipStream_t stream);
@@ -86,10 +87,9 @@ int ticc_ldmd_init(int n_energy, int n_channel, const double* W0, const double* 
   return ticc::ldmd_init(n_energy, n_channel, W0, energies, reduced_mass, Y, as_stream(stream));
 }
 
-int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, double reduced_mass, int n_sector,
-                        const double* half_steps, const double* W_start, const double* W_mid, const double* W_end,
-                        long long w_sector_stride, double* Y, void* stream) {
-  TICC_RETURN_IF(check_channels(n_channel));
+namespace {
+int sweep_args(int n_energy, int n_sector, const double* energies, const double* half_steps, const double* W_start,
+               const double* W_mid, const double* W_end, const double* Y) {
   if (n_energy < 1) return fail(TICC_ERR_VALUE, "At least one total energy is required");
   if (n_sector < 0) return fail(TICC_ERR_VALUE, "n_sector must be non-negative");
   if (!energies || !Y || (n_sector > 0 && (!half_steps || !W_start || !W_mid || !W_end)))
@@ -99,9 +99,36 @@ int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, dou
       return fail(TICC_ERR_VALUE, "radial_half_step must be positive, but got radial_half_step=%.17g", half_steps[s]);
   for (int e = 0; e < n_energy; ++e)
     if (!std::isfinite(energies[e])) return fail(TICC_ERR_VALUE, "Total energies must be finite");
+  return TICC_OK;
+}
+}  // namespace
+
+int ticc_ldmd_init_capture(int n_energy, int n_channel, const double* W0, const double* energies,
+                           double reduced_mass, double* Y, void* stream) {
+  TICC_RETURN_IF(check_channels(2 * n_channel));
+  if (n_energy < 1) return fail(TICC_ERR_VALUE, "At least one total energy is required");
+  if (!W0 || !energies || !Y) return fail(TICC_ERR_VALUE, "ldmd_init_capture: null pointer argument");
+  return ticc::ldmd_init_capture(n_energy, n_channel, W0, energies, reduced_mass, Y, as_stream(stream));
+}
+
+int ticc_ldmd_propagate_capture(int n_energy, int n_channel, const double* energies, double reduced_mass,
+                                int n_sector, const double* half_steps, const double* W_start, const double* W_mid,
+                                const double* W_end, long long w_sector_stride, double* Y, void* stream) {
+  TICC_RETURN_IF(check_channels(2 * n_channel));
+  TICC_RETURN_IF(sweep_args(n_energy, n_sector, energies, half_steps, W_start, W_mid, W_end, Y));
+  if (n_sector == 0) return TICC_OK;
+  return ticc::ldmd_propagate(n_energy, 2 * n_channel, energies, reduced_mass, n_sector, half_steps, W_start, W_mid,
+                              W_end, w_sector_stride, Y, as_stream(stream), n_channel);
+}
+
+int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, double reduced_mass, int n_sector,
+                        const double* half_steps, const double* W_start, const double* W_mid, const double* W_end,
+                        long long w_sector_stride, double* Y, void* stream) {
+  TICC_RETURN_IF(check_channels(n_channel));
+  TICC_RETURN_IF(sweep_args(n_energy, n_sector, energies, half_steps, W_start, W_mid, W_end, Y));
   if (n_sector == 0) return TICC_OK;
   return ticc::ldmd_propagate(n_energy, n_channel, energies, reduced_mass, n_sector, half_steps, W_start, W_mid, W_end,
-                              w_sector_stride, Y, as_stream(stream));
+                              w_sector_stride, Y, as_stream(stream), 0);
 }
 
 int ticc_get_smat(int n_energy, int n_channel, const double* Y, const double* B, double Rmatch,

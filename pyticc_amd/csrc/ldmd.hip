@@ -101,6 +101,10 @@ __global__ void k_form2(const double* __restrict__ X1, const int* __restrict__ p
 }
 
 // Y = (Ye + Ye^T)/2,  Ye = (Q_e + diag y1) - y2 (X2 y2).  32 x 32 tiles through LDS.
+// With half > 0 the matrices are the real form [[Re Z, -Im Z], [Im Z, Re Z]]
+// of complex symmetric N x N matrices (N = half): Z -> (Z + Z^T)/2 is then
+// (Ye + P Ye^T P)/2 with P = diag(I, -I), i.e. the transposed element enters
+// with a minus sign when exactly one of i, j lies in the second half.
 __device__ __forceinline__ double yend_elem(const double* __restrict__ X2e, const int* __restrict__ pinve,
                                             const double* __restrict__ We, const double* __restrict__ Wm, double h,
                                             const double* __restrict__ y1e, const double* __restrict__ y2e, int N,
@@ -116,7 +120,7 @@ __device__ __forceinline__ double yend_elem(const double* __restrict__ X2e, cons
 __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, const int* __restrict__ pinv,
                                                const double* __restrict__ We, const double* __restrict__ Wm,
                                                double h, const double* __restrict__ y1, const double* __restrict__ y2,
-                                               int N, int Np, double* __restrict__ Y) {
+                                               int N, int Np, int half, double* __restrict__ Y) {
   __shared__ double tile[32][33];
   const int e = blockIdx.y;
   const int tn = Np / 32;
@@ -142,7 +146,8 @@ __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, co
     double v = 0.0;
     if (i < N && j < N) {
       const double a = yend_elem(X2e, pinve, We, Wm, h, y1e, y2e, N, Np, i, j);
-      v = 0.5 * (a + tile[r][tx]);
+      const bool flip = half > 0 && ((i < half) != (j < half));
+      v = 0.5 * (flip ? a - tile[r][tx] : a + tile[r][tx]);
     }
     Y[e * n2 + (size_t)i * Np + j] = v;
   }
@@ -200,6 +205,32 @@ __global__ void k_init_inelastic(const double* __restrict__ W0, const double* __
   }
 }
 
+// Capture boundary condition in real form: Z_e = diag(z), z_i = sqrt(d) for
+// d = W0_ii - 2 mu E_e >= 0 and -i sqrt(-d) otherwise; Y_e = [[Re Z, -Im Z],
+// [Im Z, Re Z]] of size 2N.
+__global__ void k_init_capture(const double* __restrict__ W0, const double* __restrict__ E, double two_mu, int N,
+                               int nE, double* __restrict__ Y) {
+  const int N2 = 2 * N;
+  const size_t n2 = (size_t)N2 * N2;
+  const size_t total = n2 * nE;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const int e = (int)(idx / n2);
+    const size_t rem = idx % n2;
+    const int i = (int)(rem / N2);
+    const int j = (int)(rem % N2);
+    double v = 0.0;
+    if (i % N == j % N) {
+      const int c = i % N;
+      const double d = W0[(size_t)c * N + c] - two_mu * E[e];
+      const double re = d >= 0.0 ? sqrt(d) : 0.0;
+      const double im = d < 0.0 ? -sqrt(-d) : 0.0;
+      const bool top = i < N, left = j < N;
+      v = (top == left) ? re : (top ? -im : im);
+    }
+    Y[idx] = v;
+  }
+}
+
 int grid1(long long n) {
   long long b = (n + 255) / 256;
   if (b > 2048) b = 2048;
@@ -220,8 +251,20 @@ int ldmd_init(int nE, int N, const double* W0, const double* E_host, double mu, 
   return TICC_OK;
 }
 
+int ldmd_init_capture(int nE, int N, const double* W0, const double* E_host, double mu, double* Y, hipStream_t stream) {
+  void* ws = nullptr;
+  TICC_RETURN_IF(workspace(WS_MATCH, sizeof(double) * nE, &ws));
+  double* dE = static_cast<double*>(ws);
+  TICC_CHECK_HIP(hipMemcpyAsync(dE, E_host, sizeof(double) * nE, hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_init_capture, dim3(grid1((long long)nE * 4 * N * N)), dim3(256), 0, stream, W0, dE, 2.0 * mu,
+                     N, nE, Y);
+  TICC_CHECK_LAUNCH();
+  TICC_CHECK_HIP(hipStreamSynchronize(stream));
+  return TICC_OK;
+}
+
 int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const double* hs_host, const double* Ws0,
-                   const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream) {
+                   const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream, int half) {
   const int Np = round_up(N, GJ_NB);
   if (Np > MAX_NP) return fail(TICC_ERR_UNSUPPORTED, "n_channel=%d exceeds the supported maximum %d", N, MAX_NP);
   // Energy groups may run on their own streams (TICC_SWEEP_GROUPS); one group
@@ -341,7 +384,8 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
         TICC_CHECK_LAUNCH();
         double* r2 = nullptr;
         TICC_RETURN_IF(gj_invert_raw(other, r1, ne, Np, &r2, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src), I(sc.piv), st));
-        hipLaunchKernelGGL(k_form3, dim3(tn * tn, ne), dim3(256), 0, st, r2, I(sc.pinv), We, Wm, h, y1, y2, N, Np, Yg);
+        hipLaunchKernelGGL(k_form3, dim3(tn * tn, ne), dim3(256), 0, st, r2, I(sc.pinv), We, Wm, h, y1, y2, N, Np, half,
+                           Yg);
         TICC_CHECK_LAUNCH();
       }
       TICC_CHECK_HIP(hipEventRecord(ev_done[2 * g + qb], st));

@@ -47,8 +47,11 @@ def get_Smat(Ymat, Rmatch: float, Etot, reduced_mass: float, E_int, L, *, device
         raise fail(ValueError, f"Ymat must have shape {(energies.size, N, N)}, but got {tuple(Ymat.shape)}")
     if Ei.ndim != 1 or Lv.shape != Ei.shape:
         raise fail(ValueError, f"E_int and L must be one-dimensional arrays with the same shape, but got {Ei.shape} and {Lv.shape}")
-    if (isinstance(Ymat, torch.Tensor) and Ymat.is_complex()) or np.iscomplexobj(np.asarray(Ymat) if not isinstance(Ymat, torch.Tensor) else np.zeros(0)):
-        raise fail(NotImplementedError, "complex (capture-mode) matching is not supported by the MI355X backend")
     dev = _device.require_gpu(0 if device is None else device)
+    if (isinstance(Ymat, torch.Tensor) and Ymat.is_complex()) or (not isinstance(Ymat, torch.Tensor) and np.iscomplexobj(Ymat)):
+        from pyticc_amd.match.capture import get_Smat_capture_device, to_real_form
+
+        Yc = Ymat if isinstance(Ymat, torch.Tensor) else torch.from_numpy(np.asarray(Ymat, dtype=np.complex128))
+        return get_Smat_capture_device(to_real_form(Yc.to(dev)), None, Rmatch, energies, reduced_mass, Ei, Lv, dev)
     S_re, S_im, n_open = get_Smat_device(Ymat, None, Rmatch, energies, reduced_mass, Ei, Lv, dev)
     return unpack_smat(S_re, S_im, n_open)

@@ -1,11 +1,13 @@
 """LDMD propagation on the device (replaces src/pyticc/propagation/logd.py of the reference).
 
-propagate_logD (logd.py:340-398), propagate_logD_sector (232-281) and
-initialize_logD_inelastic (48-65) keep the reference signatures; the
-arithmetic runs in the HIP kernels behind ticc_ldmd_propagate /
-ticc_ldmd_init.  Results are float64 torch tensors resident on the device.
-The capture boundary condition (complex Y, logd.py:72-91) is not implemented
-on this backend and raises NotImplementedError.
+propagate_logD (logd.py:340-398), propagate_logD_sector (232-281),
+initialize_logD_inelastic (48-65) and initialize_logD_capture (72-91) keep the
+reference signatures; the arithmetic runs in the HIP kernels behind
+ticc_ldmd_propagate / ticc_ldmd_init and their capture forms.  Results are
+torch tensors resident on the device: float64 for real log derivatives,
+complex128 for capture-mode ones, which the kernels carry in the real form
+[[Re Y, -Im Y], [Im Y, Re Y]] (sums, products, inverses and real diagonal
+shifts act on it as on Y).
 """
 
 from __future__ import annotations
@@ -15,6 +17,7 @@ import torch
 
 from pyticc_amd import _backend, _device
 from pyticc_amd._log import fail
+from pyticc_amd.match.capture import block_diag_real, from_real_form, to_real_form
 
 
 def _dev(device) -> torch.device:
@@ -32,9 +35,13 @@ def _validate_square(name: str, value) -> tuple:
     return shape
 
 
+def _is_complex(value) -> bool:
+    return (isinstance(value, torch.Tensor) and value.is_complex()) or (not isinstance(value, torch.Tensor) and np.iscomplexobj(value))
+
+
 def _real(name: str, value):
-    if (isinstance(value, torch.Tensor) and value.is_complex()) or (isinstance(value, np.ndarray) and np.iscomplexobj(value)):
-        raise fail(NotImplementedError, f"{name}: complex (capture-mode) log derivatives are not supported by the MI355X backend")
+    if _is_complex(value):
+        raise fail(TypeError, f"{name} must be real")
     return value
 
 
@@ -51,6 +58,48 @@ def initialize_logD_inelastic(Wmat, *, device=None) -> torch.Tensor:
         _backend.call("ticc_ldmd_init", 1, N, _device.ptr(flat[b].contiguous()), one.ctypes.data_as(_backend._dp), 0.5,
                       _device.ptr(out[b]), _device.stream_handle(dev))
     return out.reshape(W.shape)
+
+
+def initialize_logD_capture(Wmat, *, device=None) -> torch.Tensor:
+    """Incoming-wave Y: Y_ii = sqrt(W_ii) for W_ii >= 0, -i sqrt(-W_ii) otherwise (complex128)."""
+    _validate_square("Wmat", Wmat)
+    dev = _dev(device)
+    W = _device.to_device(_real("Wmat", Wmat), dev)
+    N = W.shape[-1]
+    flat = W.reshape(-1, N, N)
+    out = torch.empty((flat.shape[0], 2 * N, 2 * N), dtype=torch.float64, device=dev)
+    zero = np.zeros(1)
+    for b in range(flat.shape[0]):
+        _backend.call("ticc_ldmd_init_capture", 1, N, _device.ptr(flat[b].contiguous()), zero.ctypes.data_as(_backend._dp),
+                      0.5, _device.ptr(out[b]), _device.stream_handle(dev))
+    return from_real_form(out).reshape(W.shape)
+
+
+def initial_logD_capture_energies(W0_base: torch.Tensor, energies, reduced_mass: float, device: torch.device) -> torch.Tensor:
+    """Real forms (n_E, 2N, 2N) of the capture-mode Y_e at the first radial point (runner.py:284-291)."""
+    N = W0_base.shape[-1]
+    E = np.ascontiguousarray(energies, dtype=np.float64)
+    Y = torch.empty((E.size, 2 * N, 2 * N), dtype=torch.float64, device=device)
+    _backend.call("ticc_ldmd_init_capture", E.size, N, _device.ptr(W0_base.contiguous()), E.ctypes.data_as(_backend._dp),
+                  float(reduced_mass), _device.ptr(Y), _device.stream_handle(device))
+    return Y
+
+
+def propagate_points_capture(Yr: torch.Tensor, energies, reduced_mass: float, half_steps, W_points: torch.Tensor,
+                             device: torch.device) -> torch.Tensor:
+    """In-place sweep of real-form capture Y (n_E, 2N, 2N) over the sectors spanned by W_points (2 n_S + 1, N, N)."""
+    E = np.ascontiguousarray(energies, dtype=np.float64)
+    hs = np.ascontiguousarray(half_steps, dtype=np.float64)
+    N = W_points.shape[-1]
+    We = block_diag_real(W_points)
+    n2 = 4 * N * N
+    base = _device.ptr(We)
+    _backend.call(
+        "ticc_ldmd_propagate_capture", E.size, N, E.ctypes.data_as(_backend._dp), float(reduced_mass), hs.size,
+        hs.ctypes.data_as(_backend._dp), base, base + 8 * n2, base + 16 * n2, 2 * n2, _device.ptr(Yr),
+        _device.stream_handle(device),
+    )
+    return Yr
 
 
 def initial_logD_energies(W0_base: torch.Tensor, energies, reduced_mass: float, device: torch.device) -> torch.Tensor:
@@ -94,14 +143,24 @@ def propagate_logD(Y_initial, total_energies, reduced_mass: float, radial_half_s
         _real(name, W)
         if tuple(W.shape) != (hs.size, shape[-1], shape[-1]):
             raise fail(ValueError, f"{name} must have shape {(hs.size, shape[-1], shape[-1])}, but got {tuple(W.shape)}")
-    Y = _device.to_device(_real("Y_initial", Y_initial), dev).clone()
     E = np.ascontiguousarray(total_energies, dtype=np.float64)
-    if Y.dim() != 3 or Y.shape[0] != E.size:
-        raise fail(ValueError, f"Y_initial must have shape (n_energy, n, n) with n_energy={E.size}, but got {tuple(Y.shape)}")
+    if len(shape) != 3 or shape[0] != E.size:
+        raise fail(ValueError, f"Y_initial must have shape (n_energy, n, n) with n_energy={E.size}, but got {shape}")
     Ws = _device.to_device(W_base_start, dev)
     Wm = _device.to_device(W_base_mid, dev)
     We = _device.to_device(W_base_end, dev)
     N = shape[-1]
+    if _is_complex(Y_initial):  # capture mode: sweep the real forms
+        Yc = Y_initial if isinstance(Y_initial, torch.Tensor) else torch.from_numpy(np.asarray(Y_initial, dtype=np.complex128))
+        Yr = to_real_form(Yc.to(dev))
+        Wr = [block_diag_real(w) for w in (Ws, Wm, We)]
+        _backend.call(
+            "ticc_ldmd_propagate_capture", E.size, N, E.ctypes.data_as(_backend._dp), float(reduced_mass), hs.size,
+            hs.ctypes.data_as(_backend._dp), _device.ptr(Wr[0]), _device.ptr(Wr[1]), _device.ptr(Wr[2]), 4 * N * N,
+            _device.ptr(Yr), _device.stream_handle(dev),
+        )
+        return from_real_form(Yr)
+    Y = _device.to_device(Y_initial, dev).clone()
     _backend.call(
         "ticc_ldmd_propagate", E.size, N, E.ctypes.data_as(_backend._dp), float(reduced_mass), hs.size,
         hs.ctypes.data_as(_backend._dp), _device.ptr(Ws), _device.ptr(Wm), _device.ptr(We), N * N, _device.ptr(Y),

@@ -1,4 +1,4 @@
-"""Window loop of the exact-CC propagation (src/pyticc/propagation/runner.py:112-322).
+"""Window loop of the propagation, inelastic or capture (src/pyticc/propagation/runner.py:112-322).
 
 Each radial window: the device contraction writes W_base at the window's new
 radial points (the shared endpoint of the previous window is reused, as in
@@ -18,7 +18,12 @@ from pyticc_amd._log import fail, logger
 from pyticc_amd.energy import get_Etot
 from pyticc_amd.propagation.device import resolve_device
 from pyticc_amd.propagation.grid import window_points
-from pyticc_amd.propagation.logd import initial_logD_energies, propagate_points
+from pyticc_amd.propagation.logd import (
+    initial_logD_capture_energies,
+    initial_logD_energies,
+    propagate_points,
+    propagate_points_capture,
+)
 
 _MAX_WINDOW = 256
 
@@ -32,10 +37,10 @@ def window_size(n_sector: int, n_grid: int, n_channel: int, memory_mb: float) ->
 
 
 def propagate(hamiltonian, Etot, radial_sectors, config) -> torch.Tensor:
-    """Y_BF at the last sector end for every energy, shape (n_E, N, N), on the device."""
+    """Y_BF at the last sector end for every energy on the device: (n_E, N, N) float64, or for
+    mode="capture" the real forms (n_E, 2N, 2N) of the complex log derivatives."""
     energies = get_Etot(Etot)
-    if config.mode != "inelastic":
-        raise fail(NotImplementedError, "capture-mode propagation is not supported by the MI355X backend")
+    capture = config.mode == "capture"
     sectors = tuple(radial_sectors)
     if not sectors:
         raise fail(RuntimeError, "Propagation produced no radial windows")
@@ -61,8 +66,10 @@ def propagate(hamiltonian, Etot, radial_sectors, config) -> torch.Tensor:
         else:
             hamiltonian.W_points(pts, dev, out=Wpts)
         if Y is None:
-            Y = initial_logD_energies(Wpts[0], energies, mu, dev)
-        propagate_points(Y, energies, mu, hs_all[w0 : w0 + len(window)], Wpts, dev)
+            init = initial_logD_capture_energies if capture else initial_logD_energies
+            Y = init(Wpts[0], energies, mu, dev)
+        sweep = propagate_points_capture if capture else propagate_points
+        sweep(Y, energies, mu, hs_all[w0 : w0 + len(window)], Wpts, dev)
         W_prev = (float(pts[-1]), Wpts[-1].clone())
         done += len(window)
         if config.print_verbose and (done == len(sectors) or done >= next_progress):

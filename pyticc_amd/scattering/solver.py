@@ -18,6 +18,7 @@ from pyticc_amd import _device
 from pyticc_amd._log import fail, logger
 from pyticc_amd.energy import get_Etot
 from pyticc_amd.match.asymptotic import get_Bmat_BF_to_SF
+from pyticc_amd.match.capture import from_real_form, get_Smat_capture_device
 from pyticc_amd.match.smatrix import get_Smat_device, unpack_smat
 from pyticc_amd.propagation.device import resolve_device
 from pyticc_amd.propagation.runner import propagate
@@ -29,7 +30,8 @@ from pyticc_amd.system import Approx, ScattSystem
 
 
 def solve(problem, Etot, potential_grid, propagation) -> ScatteringResult:
-    """Build, propagate and match one exact-CC block; returns ScatteringResult with timing."""
+    """Build, propagate and match one (J, parity) block -- exact CC, or CS / NNCC K
+    blocks -- in the inelastic or capture mode; returns the result with timing."""
     if isinstance(problem, ScattSystem):
         if not isinstance(potential_grid, PotentialGrid):
             raise fail(TypeError, "A ScattSystem solve requires a PotentialGrid")
@@ -49,22 +51,24 @@ def solve(problem, Etot, potential_grid, propagation) -> ScatteringResult:
         return _solve_blocks(hamiltonian, energies, sectors, propagation, wall0, cpu0)
     logger.info(f"Solving Jtot={basis.Jtot}, system_parity={basis.system_parity:+d}, approx=exact, channels={basis.n_channel}, energies={energies.size}")
     B, L = get_Bmat_BF_to_SF(basis)
-    Y, S_re, S_im, n_open = solve_device(hamiltonian, energies, sectors, propagation, B, L)
-    Smat = unpack_smat(S_re, S_im, n_open)
-    result = ScatteringResult(basis=basis, Etot=energies, Y_propagated=_device.to_host(Y), asymptotic_transform=B, L=L, Smat=Smat)
+    Y, Smat = solve_device(hamiltonian, energies, sectors, propagation, B, L)
+    result = ScatteringResult(basis=basis, Etot=energies, Y_propagated=Y, asymptotic_transform=B, L=L, Smat=Smat)
     timing = Timing(wall_seconds=perf_counter() - wall0, cpu_seconds=process_time() - cpu0)
     logger.info(f"Solver complete: {timing}")
     return replace(result, timing=timing)
 
 
 def solve_device(hamiltonian, energies, sectors, propagation, B, L):
-    """Device hot path: contraction + LDMD sweep + matching; returns device (Y, S_re, S_im) and n_open."""
-    if propagation.mode != "inelastic":
-        raise fail(NotImplementedError, "capture-mode propagation is not supported by the MI355X backend")
+    """Device hot path: contraction + LDMD sweep + matching.  Returns the host Y_BF
+    (float64, complex128 in capture mode) and the ragged tuple of S matrices."""
     Y = propagate(hamiltonian, energies, sectors, propagation)
     dev = resolve_device(propagation.device).torch_device
-    S_re, S_im, n_open = get_Smat_device(Y, B, sectors[-1].radial_end, energies, hamiltonian.reduced_mass, hamiltonian.basis.E_int, L, dev)
-    return Y, S_re, S_im, n_open
+    Rmatch, mu, E_int = sectors[-1].radial_end, hamiltonian.reduced_mass, hamiltonian.basis.E_int
+    if propagation.mode == "capture":
+        Smat = get_Smat_capture_device(Y, B, Rmatch, energies, mu, E_int, L, dev)
+        return _device.to_host(from_real_form(Y)), Smat
+    S_re, S_im, n_open = get_Smat_device(Y, B, Rmatch, energies, mu, E_int, L, dev)
+    return _device.to_host(Y), unpack_smat(S_re, S_im, n_open)
 
 
 def build_k_blocks(hamiltonian) -> tuple:
@@ -86,12 +90,11 @@ def _solve_blocks(hamiltonian, energies, sectors, propagation, wall0, cpu0) -> C
     for block in blocks:
         sub = hamiltonian.block(block.channel_indices)
         B, L = get_Bmat_BF_to_SF(sub.basis)
-        Y, S_re, S_im, n_open = solve_device(sub, energies, sectors, propagation, B, L)
-        Smat = unpack_smat(S_re, S_im, n_open)
+        Y, Smat = solve_device(sub, energies, sectors, propagation, B, L)
         indices = np.asarray(block.channel_indices, dtype=np.int64)
         E_int = sub.basis.E_int
         open_idx = tuple(indices[np.flatnonzero(E_int < E)] for E in energies)
-        results.append(KBlockResult(block=block, open_channel_indices=open_idx, Y_BF=_device.to_host(Y), Bmat=B, L=L,
+        results.append(KBlockResult(block=block, open_channel_indices=open_idx, Y_BF=Y, Bmat=B, L=L,
                                     Smat_asymptotic=Smat))
     timing = Timing(wall_seconds=perf_counter() - wall0, cpu_seconds=process_time() - cpu0)
     logger.info(f"Solver complete: {timing}")

@@ -85,6 +85,6 @@ def test_nonfinite_result_is_reported(gpu_device):
         propagate_logD(np.ones((1, 1, 1)), [0.1], 1.0, [0.1], W, W, W)
 
 
-def test_capture_mode_is_rejected(gpu_device):
-    with pytest.raises(NotImplementedError):
-        propagate_logD(np.array([[[1.0j]]]), [0.1], 1.0, [0.1], np.ones((1, 1, 1)), np.ones((1, 1, 1)), np.ones((1, 1, 1)))
+def test_complex_W_is_rejected(gpu_device):
+    with pytest.raises(TypeError):
+        propagate_logD(np.array([[[1.0j]]]), [0.1], 1.0, [0.1], np.ones((1, 1, 1)) * 1j, np.ones((1, 1, 1)), np.ones((1, 1, 1)))
