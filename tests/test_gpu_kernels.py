@@ -30,7 +30,8 @@ def test_dgemm_batched_matches_numpy(gpu_device, m, n, k, batch):
     np.testing.assert_allclose(_device.to_host(dC), expect, rtol=1e-12, atol=1e-12 * np.sqrt(k))
 
 
-@pytest.mark.parametrize("N,batch", [(1, 1), (2, 3), (31, 2), (33, 4), (66, 5), (100, 3), (130, 2), (200, 3), (800, 2)])
+@pytest.mark.parametrize("N,batch", [(1, 1), (2, 3), (31, 2), (33, 4), (66, 5), (100, 3), (130, 2), (200, 3), (800, 2),
+                                     (1000, 1), (1024, 2)])
 def test_invert_batched_matches_numpy(gpu_device, N, batch):
     rng = np.random.default_rng(N)
     A = rng.standard_normal((batch, N, N)) + np.eye(N) * 0.1
@@ -41,6 +42,11 @@ def test_invert_batched_matches_numpy(gpu_device, N, batch):
     cond = np.linalg.cond(A).max()
     np.testing.assert_allclose(inv, ref, rtol=0, atol=1e-13 * cond * np.abs(ref).max())
     np.testing.assert_allclose(inv @ A, np.broadcast_to(np.eye(N), A.shape), atol=1e-12 * cond)
+
+
+def test_invert_above_maximum_size_is_rejected(gpu_device):
+    with pytest.raises(NotImplementedError, match="exceeds"):
+        _backend.call("ticc_invert_batched", 1, 1025, None, None)
 
 
 def test_invert_requires_row_pivoting(gpu_device):
