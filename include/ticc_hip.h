@@ -183,6 +183,49 @@ int ticc_dgemm_batched(int m, int n, int k, double alpha, const double* A, long 
                        long long strideB, double beta, double* C, long long strideC, int batch, void* stream);
 
 /*
+ * Learned PES surrogate (no counterpart in the reference; the PES callback it
+ * stands in for is the reference's interaction routine, src/pyticc/pes/adiabatic.py:17-103).
+ * A 5 -> 256 x 4 -> 1 SiLU network in BF16 with FP32 accumulation; all
+ * matrices row-major, BF16 buffers hold raw bfloat16 bits.
+ *
+ * C[M][N] = A[M][K] . B[N][K]^T with epilogue `mode`:
+ *   0  Cf = C;
+ *   1  Z = C + bias[col] -> Cf, a = SiLU(Z) -> Cb (M x N) and CbT (N x M), BF16;
+ *   2  Cf = C + bias[col];
+ *   3  dZ = C * SiLU'(Z[row][col]) -> Cf, Cb and CbT (BF16).
+ * M, N multiples of 64, K a multiple of 16.
+ */
+int ticc_mlp_gemm(int mode, int M, int N, int K, const void* A, const void* B, const float* bias, const float* Z,
+                  float* Cf, void* Cb, void* CbT, void* stream);
+
+/* Features [r_scale/R, cos th_X, cos th_Y, cos phi, sin th_X sin th_Y cos phi] of
+ * the first n_valid rows of X4 (n_valid x 4 doubles: R, th_X, th_Y, phi) into F
+ * (n x 64 BF16, columns 5.. zero) and optionally FT (64 x n); rows past n_valid
+ * get a fixed valid geometry. */
+int ticc_mlp_features(int n, int n_valid, const double* X4, double r_scale, void* F, void* FT, void* stream);
+
+/* The same features for sample i = (R[i / n_grid], grid point i % n_grid) of a
+ * PES grid; coords column-major (n_coord x n_grid), angles in rows ix, iy, iphi. */
+int ticc_mlp_grid_features(int n, int n_valid, int n_grid, const double* R, int n_coord, const double* coords, int ix,
+                           int iy, int iphi, double r_scale, void* F, void* stream);
+
+/* Mean-square loss of column 0 of Y (n x 64) against T over n_valid rows:
+ * *loss += mean d^2, dZ / dZT column / row 0 = 2 d / n_valid (BF16), *dbias += sum of it. */
+int ticc_mlp_loss(int n, int n_valid, const float* Y, const float* T, void* dZ, void* dZT, float* dbias, float* loss,
+                  void* stream);
+
+/* out[c] += sum_r X[r][c], X (M x N), N a multiple of 64. */
+int ticc_mlp_colsum(int M, int N, const float* X, float* out, void* stream);
+
+/* One Adam step (bias-corrected, step >= 1) of an FP32 rows x cols tensor; writes
+ * the BF16 copy bf and the BF16 transpose bfT when they are not NULL. */
+int ticc_mlp_adam(int rows, int cols, float* theta, const float* grad, float* m, float* v, float lr, float beta1,
+                  float beta2, float eps, int step, void* bf, void* bfT, void* stream);
+
+/* BF16 copy and transpose of an FP32 rows x cols tensor. */
+int ticc_mlp_to_bf16(int rows, int cols, const float* x, void* bf, void* bfT, void* stream);
+
+/*
  * Instrumentation (no counterpart in the reference): live HIP-event timing of
  * the sweep's dominant kernel, the fused Gauss-Jordan launch (one pair's
  * rank-64 update beside the next pair's panel chain).  Between begin and end,

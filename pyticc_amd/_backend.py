@@ -51,6 +51,17 @@ _SIGNATURES = {
     "ticc_profile_update_begin": ([_c_int, _c_int], _c_int),
     "ticc_profile_update_end": ([_dp, _ip, _dp, _dp], _c_int),
     "ticc_dgemm_batched": ([_c_int, _c_int, _c_int, _c_d, _vp, _c_ll, _vp, _c_ll, _c_d, _vp, _c_ll, _c_int, _vp], _c_int),
+    "ticc_mlp_gemm": ([_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "ticc_mlp_features": ([_c_int, _c_int, _vp, _c_d, _vp, _vp, _vp], _c_int),
+    "ticc_mlp_grid_features": ([_c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _c_int, _c_d, _vp, _vp], _c_int),
+    "ticc_mlp_loss": ([_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "ticc_mlp_colsum": ([_c_int, _c_int, _vp, _vp, _vp], _c_int),
+    "ticc_mlp_adam": (
+        [_c_int, _c_int, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _c_int, _vp,
+         _vp, _vp],
+        _c_int,
+    ),
+    "ticc_mlp_to_bf16": ([_c_int, _c_int, _vp, _vp, _vp, _vp], _c_int),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)

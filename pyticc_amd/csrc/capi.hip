@@ -11,6 +11,17 @@ int contract_w(int N, int n_ang, int n_block, const int* block_offsets, const in
                const double* Eint, double mu, double* W, hipStream_t stream);
 int ldmd_init(int nE, int N, const double* W0, const double* E_host, double mu, double* Y, hipStream_t stream);
 int ldmd_init_capture(int nE, int N, const double* W0, const double* E_host, double mu, double* Y, hipStream_t stream);
+int mlp_gemm(int mode, int M, int N, int K, const void* A, const void* B, const float* bias, const float* Z, float* Cf,
+             void* Cb, void* CbT, hipStream_t stream);
+int mlp_features(int n, int n_valid, const double* X4, double r_scale, void* F, void* FT, hipStream_t stream);
+int mlp_grid_features(int n, int n_valid, int n_grid, const double* R, int n_coord, const double* coords, int ix, int iy,
+                      int iphi, double r_scale, void* F, hipStream_t stream);
+int mlp_loss(int n, int n_valid, const float* Y, const float* T, void* dZ, void* dZT, float* dbias, float* loss,
+             hipStream_t stream);
+int mlp_colsum(int M, int N, const float* X, float* out, hipStream_t stream);
+int mlp_adam(int rows, int cols, float* theta, const float* grad, float* m, float* v, float lr, float b1, float b2,
+             float eps, int step, void* bf, void* bfT, hipStream_t stream);
+int mlp_to_bf16(int rows, int cols, const float* x, void* bf, void* bfT, hipStream_t stream);
 int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const double* hs_host, const double* Ws0,
                    const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream, int half);
 int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, const double* E_host, double mu,
@@ -197,6 +208,53 @@ int ticc_dgemm_batched(int m, int n, int k, double alpha, const double* A, long 
   if (!A || !B || !C) return fail(TICC_ERR_VALUE, "dgemm: null pointer argument");
   return ticc::dgemm_batched(false, false, m, n, k, alpha, A, k, strideA, B, n, strideB, beta, C, n, strideC, batch,
                              as_stream(stream));
+}
+
+int ticc_mlp_gemm(int mode, int M, int N, int K, const void* A, const void* B, const float* bias, const float* Z,
+                  float* Cf, void* Cb, void* CbT, void* stream) {
+  if (!A || !B) return fail(TICC_ERR_VALUE, "mlp_gemm: null operand");
+  if ((mode == 1 || mode == 2) && !bias) return fail(TICC_ERR_VALUE, "mlp_gemm: bias required");
+  if (mode == 3 && !Z) return fail(TICC_ERR_VALUE, "mlp_gemm: pre-activations required");
+  if (!Cf || ((mode == 1 || mode == 3) && (!Cb || !CbT))) return fail(TICC_ERR_VALUE, "mlp_gemm: null output");
+  return ticc::mlp_gemm(mode, M, N, K, A, B, bias, Z, Cf, Cb, CbT, as_stream(stream));
+}
+
+int ticc_mlp_features(int n, int n_valid, const double* X4, double r_scale, void* F, void* FT, void* stream) {
+  if (n <= 0 || n_valid < 0 || n_valid > n || !F || (n_valid > 0 && !X4))
+    return fail(TICC_ERR_VALUE, "mlp_features: invalid arguments");
+  return ticc::mlp_features(n, n_valid, X4, r_scale, F, FT, as_stream(stream));
+}
+
+int ticc_mlp_grid_features(int n, int n_valid, int n_grid, const double* R, int n_coord, const double* coords, int ix,
+                           int iy, int iphi, double r_scale, void* F, void* stream) {
+  if (n <= 0 || n_valid < 0 || n_valid > n || n_grid <= 0 || !F || !R || !coords || ix < 0 || iy < 0 || iphi < 0 ||
+      ix >= n_coord || iy >= n_coord || iphi >= n_coord)
+    return fail(TICC_ERR_VALUE, "mlp_grid_features: invalid arguments");
+  return ticc::mlp_grid_features(n, n_valid, n_grid, R, n_coord, coords, ix, iy, iphi, r_scale, F, as_stream(stream));
+}
+
+int ticc_mlp_loss(int n, int n_valid, const float* Y, const float* T, void* dZ, void* dZT, float* dbias, float* loss,
+                  void* stream) {
+  if (n <= 0 || n_valid <= 0 || n_valid > n || !Y || !T || !dZ || !dZT || !dbias || !loss)
+    return fail(TICC_ERR_VALUE, "mlp_loss: invalid arguments");
+  return ticc::mlp_loss(n, n_valid, Y, T, dZ, dZT, dbias, loss, as_stream(stream));
+}
+
+int ticc_mlp_colsum(int M, int N, const float* X, float* out, void* stream) {
+  if (M <= 0 || N <= 0 || !X || !out) return fail(TICC_ERR_VALUE, "mlp_colsum: invalid arguments");
+  return ticc::mlp_colsum(M, N, X, out, as_stream(stream));
+}
+
+int ticc_mlp_adam(int rows, int cols, float* theta, const float* grad, float* m, float* v, float lr, float beta1,
+                  float beta2, float eps, int step, void* bf, void* bfT, void* stream) {
+  if (rows <= 0 || cols <= 0 || step < 1 || !theta || !grad || !m || !v)
+    return fail(TICC_ERR_VALUE, "mlp_adam: invalid arguments");
+  return ticc::mlp_adam(rows, cols, theta, grad, m, v, lr, beta1, beta2, eps, step, bf, bfT, as_stream(stream));
+}
+
+int ticc_mlp_to_bf16(int rows, int cols, const float* x, void* bf, void* bfT, void* stream) {
+  if (rows <= 0 || cols <= 0 || !x) return fail(TICC_ERR_VALUE, "mlp_to_bf16: invalid arguments");
+  return ticc::mlp_to_bf16(rows, cols, x, bf, bfT, as_stream(stream));
 }
 
 int ticc_profile_update_begin(int sample_every, int max_samples) {
