@@ -279,3 +279,65 @@ class PESSurrogate:
 
         return PESWrapper(interaction=lambda R, c: many(np.array([R]), c)[0], monomer_X=exact.monomer_X,
                           monomer_Y=exact.monomer_Y, interaction_many=many)
+
+
+def benchmark_rotor_rotor(n_samples: int = 2_000_000, epochs: int = 30, batch: int = 8192, lr: float = 2e-3,
+                          seed: int = 5, device: int = 0) -> dict:
+    """Train the surrogate on samples of the two-rotor benchmark PES at the points its
+    PotentialGrid evaluates (4001 radial points x 15 x 15 x 12 angles, 4-40 bohr),
+    then compare the grid it produces (with the exact-PES fallback) and a reduced
+    two-rotor solve against the exact PES."""
+    import pyticc_amd as ticc
+    from dataclasses import replace
+    from pyticc_amd import synthetic
+
+    out = {}
+    w = synthetic.rotor_rotor()
+    quad = w.quadrature
+    t0 = perf_counter()
+    grid = ticc.prepare_potential(w.system, w.boundaries, w.half_steps, **quad)
+    out["exact_grid_s"] = perf_counter() - t0
+    V = grid.values.reshape(grid.radial_points.size, -1)
+    ang = np.stack(np.meshgrid(np.arccos(grid.coordinate("cos_theta_X")), np.arccos(grid.coordinate("cos_theta_Y")),
+                               grid.coordinate("phi"), indexing="ij"), axis=-1).reshape(-1, 3)
+    rng = np.random.default_rng(seed)
+    pick = rng.choice(V.size, size=min(n_samples, V.size), replace=False)
+    ri, gi = np.divmod(pick, V.shape[1])
+    X4 = np.column_stack([grid.radial_points[ri], ang[gi]])
+    net = PESSurrogate(seed=seed, device=device)
+    t0 = perf_counter()
+    hist = net.fit(X4, V.reshape(-1)[pick], epochs=epochs, batch=batch, lr=lr)
+    torch.cuda.synchronize(net.dev)
+    out["train_s"] = perf_counter() - t0
+    out["samples"] = int(pick.size)
+    out["epochs"] = epochs
+    out["loss_first_last"] = (hist[0], hist[-1])
+    rest = np.setdiff1d(rng.choice(V.size, size=200_000, replace=False), pick)
+    ri, gi = np.divmod(rest, V.shape[1])
+    Vt = V.reshape(-1)[rest]
+    keep = np.abs(Vt) <= 0.05
+    pred = net.predict(np.column_stack([grid.radial_points[ri], ang[gi]])[keep])
+    out["heldout_rms_eh"] = float(np.sqrt(np.mean((pred - Vt[keep]) ** 2)))
+    out["heldout_V_rms_eh"] = float(np.sqrt(np.mean(Vt[keep] ** 2)))
+    surrogate = net.pes(w.system.potential)
+    t0 = perf_counter()
+    grid_s = ticc.prepare_potential(replace(w.system, potential=surrogate), w.boundaries, w.half_steps, **quad)
+    out["surrogate_grid_s"] = perf_counter() - t0
+    out["exact_fallback_fraction"] = net.fallback_points / max(1, net.fallback_points + net.surrogate_points)
+    dV = np.abs(grid_s.values - grid.values)
+    out["grid_max_abs_dV_eh"] = float(dV.max())
+    # reduced two-rotor solve on both grids (same radial grid and quadrature)
+    small = synthetic.rotor_rotor(jmax_X=2, jmax_Y=3, Jtot=2, n_energy=4)
+    prop = ticc.Propagation(device="gpu")
+    ref = ticc.solve(small.system, small.energies, grid, prop)
+    sur = ticc.solve(small.system, small.energies, grid_s, prop)
+    out["reduced_N"] = int(small.system.basis.n_channel)
+    out["reduced_max_abs_dS"] = float(max(np.abs(a - b).max() for a, b in zip(ref.Smat, sur.Smat) if a.size))
+    return out
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+
+    print(json.dumps(benchmark_rotor_rotor(epochs=int(sys.argv[1]) if len(sys.argv) > 1 else 30)), flush=True)
