@@ -113,6 +113,22 @@ int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, dou
                         long long w_sector_stride, double* Y, void* stream);
 
 /*
+ * LDMD sweep of several channel blocks in one batch: the CS / NNCC K blocks of
+ * one (J, parity) problem (src/pyticc/propagation/runner.py:326-374, which
+ * propagates all blocks through each radial window together), or any blocks of
+ * one common size.  Each of the n_block blocks has its own W_base at
+ * W_x + b * w_block_stride (sector s at + s * w_sector_stride) and is swept at
+ * the same n_energy energies; Y is (n_block * n_energy, N, N), block-major.
+ * Blocks smaller than N are padded by the caller with decoupled closed channels
+ * (zero coupling, a large positive diagonal), which leaves their own channels'
+ * sweep unchanged.
+ */
+int ticc_ldmd_propagate_blocks(int n_block, int n_energy, int n_channel, const double* energies, double reduced_mass,
+                               int n_sector, const double* half_steps, const double* W_start, const double* W_mid,
+                               const double* W_end, long long w_sector_stride, long long w_block_stride, double* Y,
+                               void* stream);
+
+/*
  * Capture boundary condition (Propagation(mode="capture")): complex symmetric
  * log derivatives Z carried in the real form [[Re Z, -Im Z], [Im Z, Re Z]]
  * (2N x 2N), on which sums, products, inverses and real diagonal shifts act as

@@ -1,6 +1,8 @@
 // extern "C" entry points of libticc_hip.so: argument validation and dispatch.
 #include <cmath>
 
+#include <vector>
+
 #include "ticc_common.h"
 
 namespace ticc {
@@ -23,7 +25,8 @@ int mlp_adam(int rows, int cols, float* theta, const float* grad, float* m, floa
              float eps, int step, void* bf, void* bfT, hipStream_t stream);
 int mlp_to_bf16(int rows, int cols, const float* x, void* bf, void* bfT, hipStream_t stream);
 int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const double* hs_host, const double* Ws0,
-                   const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream, int half);
+                   const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream, int half,
+                   int n_block, long long gstride);
 int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, const double* E_host, double mu,
              const double* Eint_host, const double* L_host, double* S_re, double* S_im, int* n_open,
              hipStream_t stream);
@@ -129,7 +132,7 @@ int ticc_ldmd_propagate_capture(int n_energy, int n_channel, const double* energ
   TICC_RETURN_IF(sweep_args(n_energy, n_sector, energies, half_steps, W_start, W_mid, W_end, Y));
   if (n_sector == 0) return TICC_OK;
   return ticc::ldmd_propagate(n_energy, 2 * n_channel, energies, reduced_mass, n_sector, half_steps, W_start, W_mid,
-                              W_end, w_sector_stride, Y, as_stream(stream), n_channel);
+                              W_end, w_sector_stride, Y, as_stream(stream), n_channel, 1, 0);
 }
 
 int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, double reduced_mass, int n_sector,
@@ -139,7 +142,23 @@ int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, dou
   TICC_RETURN_IF(sweep_args(n_energy, n_sector, energies, half_steps, W_start, W_mid, W_end, Y));
   if (n_sector == 0) return TICC_OK;
   return ticc::ldmd_propagate(n_energy, n_channel, energies, reduced_mass, n_sector, half_steps, W_start, W_mid, W_end,
-                              w_sector_stride, Y, as_stream(stream), 0);
+                              w_sector_stride, Y, as_stream(stream), 0, 1, 0);
+}
+
+int ticc_ldmd_propagate_blocks(int n_block, int n_energy, int n_channel, const double* energies, double reduced_mass,
+                               int n_sector, const double* half_steps, const double* W_start, const double* W_mid,
+                               const double* W_end, long long w_sector_stride, long long w_block_stride, double* Y,
+                               void* stream) {
+  TICC_RETURN_IF(check_channels(n_channel));
+  if (n_block < 1) return fail(TICC_ERR_VALUE, "n_block must be positive");
+  TICC_RETURN_IF(sweep_args(n_energy, n_sector, energies, half_steps, W_start, W_mid, W_end, Y));
+  if ((long long)n_block * n_energy > (1LL << 20)) return fail(TICC_ERR_VALUE, "too many block-energy pairs");
+  if (n_sector == 0) return TICC_OK;
+  std::vector<double> all((size_t)n_block * n_energy);
+  for (int b = 0; b < n_block; ++b)
+    for (int e = 0; e < n_energy; ++e) all[(size_t)b * n_energy + e] = energies[e];
+  return ticc::ldmd_propagate(n_block * n_energy, n_channel, all.data(), reduced_mass, n_sector, half_steps, W_start,
+                              W_mid, W_end, w_sector_stride, Y, as_stream(stream), 0, n_block, w_block_stride);
 }
 
 int ticc_get_smat(int n_energy, int n_channel, const double* Y, const double* B, double Rmatch,

@@ -37,15 +37,22 @@ __device__ __forceinline__ void reference_values(double h, double p2, double* y1
   }
 }
 
+// Blocks: batch item item0 + e belongs to block (item0 + e) / per, whose W_base
+// lies gstride elements after the previous block's (gstride = 0: one block).
+__device__ __forceinline__ size_t block_off(int item0, int e, int per, long long gstride) {
+  return (size_t)((item0 + e) / per) * (size_t)gstride;
+}
+
 __global__ void k_refvals(double h, const double* __restrict__ Wm, int N, int Np, const double* __restrict__ E,
-                          double two_mu, int nE, double* __restrict__ y1, double* __restrict__ y2) {
+                          double two_mu, int nE, double* __restrict__ y1, double* __restrict__ y2, int item0, int per,
+                          long long gstride) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nE * Np) return;
   const int e = idx / Np;
   const int i = idx % Np;
   double a = 0.0, b = 0.0;
   if (i < N) {
-    const double p2 = Wm[(size_t)i * N + i] - two_mu * E[e];
+    const double p2 = Wm[block_off(item0, e, per, gstride) + (size_t)i * N + i] - two_mu * E[e];
     reference_values(h, p2, &a, &b);
   }
   y1[idx] = a;
@@ -53,10 +60,13 @@ __global__ void k_refvals(double h, const double* __restrict__ Wm, int N, int Np
 }
 
 // A1 = (Y + Q_s) + diag y1, identity padding.
-__global__ void k_form1(const double* __restrict__ Y, const double* __restrict__ Ws, const double* __restrict__ Wm,
-                        double h, const double* __restrict__ y1, int N, int Np, double* __restrict__ A) {
+__global__ void k_form1(const double* __restrict__ Y, const double* __restrict__ Ws0, const double* __restrict__ Wm0,
+                        double h, const double* __restrict__ y1, int N, int Np, double* __restrict__ A, int item0, int per,
+                        long long gstride) {
   const int e = blockIdx.y;
   const size_t n2 = (size_t)Np * Np;
+  const double* Ws = Ws0 + block_off(item0, e, per, gstride);
+  const double* Wm = Wm0 + block_off(item0, e, per, gstride);
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n2; idx += (size_t)gridDim.x * blockDim.x) {
     const int i = (int)(idx / Np);
     const int j = (int)(idx % Np);
@@ -75,11 +85,12 @@ __global__ void k_form1(const double* __restrict__ Y, const double* __restrict__
 }
 
 // A2 = (Ym + Q_m) + diag y1,  Ym = (Q_m + diag y1) - y2 (X1 y2).
-__global__ void k_form2(const double* __restrict__ X1, const int* __restrict__ pinv, const double* __restrict__ Qm,
+__global__ void k_form2(const double* __restrict__ X1, const int* __restrict__ pinv, const double* __restrict__ Qm0,
                         const double* __restrict__ y1, const double* __restrict__ y2, int N, int Np,
-                        double* __restrict__ A) {
+                        double* __restrict__ A, int item0, int per, long long qstride) {
   const int e = blockIdx.y;
   const size_t n2 = (size_t)Np * Np;
+  const double* Qm = Qm0 + block_off(item0, e, per, qstride);
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n2; idx += (size_t)gridDim.x * blockDim.x) {
     const int i = (int)(idx / Np);
     const int j = (int)(idx % Np);
@@ -120,9 +131,12 @@ __device__ __forceinline__ double yend_elem(const double* __restrict__ X2e, cons
 __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, const int* __restrict__ pinv,
                                                const double* __restrict__ We, const double* __restrict__ Wm,
                                                double h, const double* __restrict__ y1, const double* __restrict__ y2,
-                                               int N, int Np, int half, double* __restrict__ Y) {
+                                               int N, int Np, int half, double* __restrict__ Y, int item0, int per,
+                                               long long gstride) {
   __shared__ double tile[32][33];
   const int e = blockIdx.y;
+  We += block_off(item0, e, per, gstride);
+  Wm += block_off(item0, e, per, gstride);
   const int tn = Np / 32;
   const int ti = blockIdx.x / tn;
   const int tj = blockIdx.x % tn;
@@ -154,11 +168,13 @@ __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, co
 }
 
 // M_s = I - (h_s^2 T_mid)/6 for a chunk of sectors.
+// (grid.y = n_block * nc: item blockIdx.y = block * nc + sector)
 __global__ void k_qm_form(const double* __restrict__ Wm0, long long stride, const double* __restrict__ hs, int N,
-                          int Np, double* __restrict__ M) {
-  const int s = blockIdx.y;
+                          int Np, double* __restrict__ M, int nc, long long gstride) {
+  const int s = blockIdx.y % nc;
+  const int g = blockIdx.y / nc;
   const double h = hs[s];
-  const double* Wm = Wm0 + s * stride;
+  const double* Wm = Wm0 + (size_t)g * gstride + s * stride;
   const size_t n2 = (size_t)Np * Np;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n2; idx += (size_t)gridDim.x * blockDim.x) {
     const int i = (int)(idx / Np);
@@ -171,22 +187,25 @@ __global__ void k_qm_form(const double* __restrict__ Wm0, long long stride, cons
     } else {
       v = (i == j) ? 1.0 : 0.0;
     }
-    M[s * n2 + idx] = v;
+    M[blockIdx.y * n2 + idx] = v;
   }
 }
 
 // Q_m = (4/h) (M^-1 - I).
+// (grid.y = n_block * nc as k_qm_form; block g's sector s goes to Qm[g * qcap + s])
 __global__ void k_qm_finish(const double* __restrict__ Xi, const int* __restrict__ pinv, const double* __restrict__ hs,
-                            int N, int Np, double* __restrict__ Qm) {
-  const int s = blockIdx.y;
+                            int N, int Np, double* __restrict__ Qm, int nc, int qcap) {
+  const int b = blockIdx.y;
+  const int s = b % nc;
+  const int g = b / nc;
   const double c = 4.0 / hs[s];
   const size_t n2 = (size_t)Np * Np;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n2; idx += (size_t)gridDim.x * blockDim.x) {
     const int i = (int)(idx / Np);
     const int j = (int)(idx % Np);
     double v = 0.0;
-    if (i < N && j < N) v = c * (Xi[s * n2 + (size_t)i * Np + pinv[(size_t)s * Np + j]] - ((i == j) ? 1.0 : 0.0));
-    Qm[s * n2 + idx] = v;
+    if (i < N && j < N) v = c * (Xi[b * n2 + (size_t)i * Np + pinv[(size_t)b * Np + j]] - ((i == j) ? 1.0 : 0.0));
+    Qm[((size_t)g * qcap + s) * n2 + idx] = v;
   }
 }
 
@@ -264,9 +283,14 @@ int ldmd_init_capture(int nE, int N, const double* W0, const double* E_host, dou
 }
 
 int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const double* hs_host, const double* Ws0,
-                   const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream, int half) {
+                   const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream, int half,
+                   int n_block, long long gstride) {
   const int Np = round_up(N, GJ_NB);
   if (Np > MAX_NP) return fail(TICC_ERR_UNSUPPORTED, "n_channel=%d exceeds the supported maximum %d", N, MAX_NP);
+  if (n_block < 1 || nE % n_block) return fail(TICC_ERR_VALUE, "the batch must hold the same energies for every block");
+  // batch item b belongs to block b / per; each block's Q_mid chunk holds qc sectors
+  const int per_block = nE / n_block;
+  const int qc = n_block == 1 ? QCHUNK : (QCHUNK / n_block > 0 ? QCHUNK / n_block : 1);
   // Energy groups may run on their own streams (TICC_SWEEP_GROUPS); one group
   // is the default: the panel workgroup needs a whole CU's registers, so it
   // does not co-run with another group's update.
@@ -284,8 +308,8 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
     return o;
   };
   const size_t oY = take(sizeof(double) * nE * n2);
-  const size_t oQ = take(sizeof(double) * 2 * QCHUNK * n2);
-  const size_t oM = take(sizeof(double) * 2 * QCHUNK * n2);
+  const size_t oQ = take(sizeof(double) * 2 * n_block * qc * n2);
+  const size_t oM = take(sizeof(double) * 2 * n_block * qc * n2);
   const size_t oE = take(sizeof(double) * nE);
   const size_t oH = take(sizeof(double) * nS);
   const size_t oFlag = take(sizeof(int));
@@ -305,7 +329,7 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
     sc.y2 = yv ? take(sizeof(double) * batch * Np) : 0;
     return sc;
   };
-  const Scratch sm = scratch(QCHUNK, false);
+  const Scratch sm = scratch(n_block * qc, false);
   Scratch sg[MAX_GROUPS];
   for (int g = 0; g < G; ++g) sg[g] = scratch(per, true);
   void* base = nullptr;
@@ -316,7 +340,7 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   double* Yp = D(oY);
   double* Qm = D(oQ);
   double* M0 = D(oM);
-  double* M1 = M0 + QCHUNK * n2;
+  double* M1 = M0 + (size_t)n_block * qc * n2;
   double* dE = D(oE);
   double* dH = D(oH);
   int* flag = I(oFlag);
@@ -336,20 +360,23 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   const double two_mu = 2.0 * mu;
   const int gelem = grid1((long long)n2);
   const int tn = Np / 32;
-  const int n_chunk = (nS + QCHUNK - 1) / QCHUNK;
+  const int n_chunk = (nS + qc - 1) / qc;
+  const long long qstride = (long long)qc * n2;  // one block's Q_mid chunk
   for (int ci = 0; ci < n_chunk; ++ci) {
-    const int s0 = ci * QCHUNK;
-    const int nc = (nS - s0) < QCHUNK ? (nS - s0) : QCHUNK;
+    const int s0 = ci * qc;
+    const int nc = (nS - s0) < qc ? (nS - s0) : qc;
     const int qb = ci & 1;
-    double* Qbuf = Qm + (size_t)qb * QCHUNK * n2;
+    double* Qbuf = Qm + (size_t)qb * n_block * qc * n2;
     if (ci >= 2)  // the groups are done with the chunk that last used this buffer
       for (int g = 0; g < G; ++g) TICC_CHECK_HIP(hipStreamWaitEvent(stream, ev_done[2 * g + qb], 0));
-    hipLaunchKernelGGL(k_qm_form, dim3(gelem, nc), dim3(256), 0, stream, Wm0 + s0 * wstride, wstride, dH + s0, N, Np,
-                       M0);
+    hipLaunchKernelGGL(k_qm_form, dim3(gelem, n_block * nc), dim3(256), 0, stream, Wm0 + s0 * wstride, wstride, dH + s0,
+                       N, Np, M0, nc, gstride);
     TICC_CHECK_LAUNCH();
     double* res = nullptr;
-    TICC_RETURN_IF(gj_invert_raw(M0, M1, nc, Np, &res, I(sm.perm), I(sm.pinv), D(sm.P), I(sm.src), I(sm.piv), stream));
-    hipLaunchKernelGGL(k_qm_finish, dim3(gelem, nc), dim3(256), 0, stream, res, I(sm.pinv), dH + s0, N, Np, Qbuf);
+    TICC_RETURN_IF(
+        gj_invert_raw(M0, M1, n_block * nc, Np, &res, I(sm.perm), I(sm.pinv), D(sm.P), I(sm.src), I(sm.piv), stream));
+    hipLaunchKernelGGL(k_qm_finish, dim3(gelem, n_block * nc), dim3(256), 0, stream, res, I(sm.pinv), dH + s0, N, Np,
+                       Qbuf, nc, qc);
     TICC_CHECK_LAUNCH();
     TICC_CHECK_HIP(hipEventRecord(ev_qm[qb], stream));
 
@@ -372,20 +399,21 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
         const double* Wm = Wm0 + s * wstride;
         const double* We = We0 + s * wstride;
         hipLaunchKernelGGL(k_refvals, dim3((ne * Np + 255) / 256), dim3(256), 0, st, h, Wm, N, Np, dE + e0, two_mu, ne,
-                           y1, y2);
+                           y1, y2, e0, per_block, gstride);
         TICC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_form1, dim3(gelem, ne), dim3(256), 0, st, Yg, Ws, Wm, h, y1, N, Np, A0);
+        hipLaunchKernelGGL(k_form1, dim3(gelem, ne), dim3(256), 0, st, Yg, Ws, Wm, h, y1, N, Np, A0, e0, per_block,
+                           gstride);
         TICC_CHECK_LAUNCH();
         double* r1 = nullptr;
         TICC_RETURN_IF(gj_invert_raw(A0, A1, ne, Np, &r1, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src), I(sc.piv), st));
         double* other = (r1 == A0) ? A1 : A0;
         hipLaunchKernelGGL(k_form2, dim3(gelem, ne), dim3(256), 0, st, r1, I(sc.pinv), Qbuf + q * n2, y1, y2, N, Np,
-                           other);
+                           other, e0, per_block, qstride);
         TICC_CHECK_LAUNCH();
         double* r2 = nullptr;
         TICC_RETURN_IF(gj_invert_raw(other, r1, ne, Np, &r2, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src), I(sc.piv), st));
         hipLaunchKernelGGL(k_form3, dim3(tn * tn, ne), dim3(256), 0, st, r2, I(sc.pinv), We, Wm, h, y1, y2, N, Np, half,
-                           Yg);
+                           Yg, e0, per_block, gstride);
         TICC_CHECK_LAUNCH();
       }
       TICC_CHECK_HIP(hipEventRecord(ev_done[2 * g + qb], st));

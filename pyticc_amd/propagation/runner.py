@@ -14,6 +14,7 @@ from time import perf_counter
 import numpy as np
 import torch
 
+from pyticc_amd import _backend, _device
 from pyticc_amd._log import fail, logger
 from pyticc_amd.energy import get_Etot
 from pyticc_amd.propagation.device import resolve_device
@@ -77,3 +78,53 @@ def propagate(hamiltonian, Etot, radial_sectors, config) -> torch.Tensor:
             while next_progress <= done:
                 next_progress += progress_every
     return Y
+
+
+def propagate_blocks(hamiltonians, Etot, radial_sectors, config) -> list:
+    """Y_BF of several channel blocks swept together (runner.py:326-374): every block's
+    W_base is contracted per radial window, padded to the largest block with decoupled
+    closed channels, and the whole (block x energy) batch goes through one device
+    sweep.  Returns one (n_E, N_b, N_b) device tensor per block."""
+    energies = get_Etot(Etot)
+    if config.mode != "inelastic":
+        raise fail(ValueError, "Batched block propagation is for the inelastic mode")
+    sectors = tuple(radial_sectors)
+    if not sectors:
+        raise fail(RuntimeError, "Propagation produced no radial windows")
+    dev = resolve_device(config.device).torch_device
+    sizes = [h.basis.n_channel for h in hamiltonians]
+    nb, Nmax = len(hamiltonians), max(sizes)
+    mu = hamiltonians[0].reduced_mass
+    E = np.ascontiguousarray(energies, dtype=np.float64)
+    pad = 2.0 * mu * float(np.max(np.abs(E))) + 1.0e4  # closed, decoupled padding channels
+    size = window_size(len(sectors), hamiltonians[0].potential_grid_size, Nmax * int(np.ceil(np.sqrt(nb))),
+                       config.memory_mb)
+    hs_all = np.asarray([s.radial_half_step for s in sectors])
+    Y = None
+    W_prev = None
+    for w0 in range(0, len(sectors), size):
+        window = sectors[w0 : w0 + size]
+        pts = window_points(window)
+        Wall = torch.zeros((nb, pts.size, Nmax, Nmax), dtype=torch.float64, device=dev)
+        reuse = W_prev is not None and np.isclose(pts[0], W_prev[0], rtol=0.0, atol=1.0e-12)
+        for b, (ham, n) in enumerate(zip(hamiltonians, sizes)):
+            Wb = torch.empty((pts.size, n, n), dtype=torch.float64, device=dev)
+            if reuse:
+                Wb[0].copy_(W_prev[1][b, :n, :n])
+                ham.W_points(pts[1:], dev, out=Wb[1:])
+            else:
+                ham.W_points(pts, dev, out=Wb)
+            Wall[b, :, :n, :n] = Wb
+            if n < Nmax:
+                idx = torch.arange(n, Nmax, device=dev)
+                Wall[b, :, idx, idx] = pad
+        if Y is None:
+            Y = torch.cat([initial_logD_energies(Wall[b, 0], E, mu, dev) for b in range(nb)])
+        n2 = Nmax * Nmax
+        base = _device.ptr(Wall)
+        _backend.call("ticc_ldmd_propagate_blocks", nb, E.size, Nmax, E.ctypes.data_as(_backend._dp), float(mu),
+                      len(window), np.ascontiguousarray(hs_all[w0 : w0 + len(window)]).ctypes.data_as(_backend._dp),
+                      base, base + 8 * n2, base + 16 * n2, 2 * n2, pts.size * n2, _device.ptr(Y),
+                      _device.stream_handle(dev))
+        W_prev = (float(pts[-1]), Wall[:, -1].clone())
+    return [Y[b * E.size : (b + 1) * E.size, :n, :n].contiguous() for b, n in enumerate(sizes)]

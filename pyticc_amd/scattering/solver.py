@@ -21,7 +21,7 @@ from pyticc_amd.match.asymptotic import get_Bmat_BF_to_SF
 from pyticc_amd.match.capture import from_real_form, get_Smat_capture_device
 from pyticc_amd.match.smatrix import get_Smat_device, unpack_smat
 from pyticc_amd.propagation.device import resolve_device
-from pyticc_amd.propagation.runner import propagate
+from pyticc_amd.propagation.runner import propagate, propagate_blocks
 from pyticc_amd.basis.kblock import build_cs_blocks, build_nncc_blocks
 from pyticc_amd.result import CoupledStatesResult, KBlockResult, ScatteringResult, Timing
 from pyticc_amd.scattering.hamiltonian import ScattHamiltonian, build_hamiltonian
@@ -62,6 +62,11 @@ def solve_device(hamiltonian, energies, sectors, propagation, B, L):
     """Device hot path: contraction + LDMD sweep + matching.  Returns the host Y_BF
     (float64, complex128 in capture mode) and the ragged tuple of S matrices."""
     Y = propagate(hamiltonian, energies, sectors, propagation)
+    return match_device(hamiltonian, Y, energies, sectors, propagation, B, L)
+
+
+def match_device(hamiltonian, Y, energies, sectors, propagation, B, L):
+    """Matching of propagated device log derivatives; returns (host Y_BF, S tuple)."""
     dev = resolve_device(propagation.device).torch_device
     Rmatch, mu, E_int = sectors[-1].radial_end, hamiltonian.reduced_mass, hamiltonian.basis.E_int
     if propagation.mode == "capture":
@@ -86,11 +91,17 @@ def _solve_blocks(hamiltonian, energies, sectors, propagation, wall0, cpu0) -> C
     blocks = build_k_blocks(hamiltonian)
     logger.info(f"Solving Jtot={basis.Jtot}, system_parity={basis.system_parity:+d}, approx={hamiltonian.approx.value}, "
                 f"channels={basis.n_channel}, blocks={len(blocks)}, energies={energies.size}")
+    subs = [hamiltonian.block(block.channel_indices) for block in blocks]
+    # inelastic K blocks go through one batched sweep (runner.py:326-374 propagates
+    # all blocks through each radial window together)
+    Ys = propagate_blocks(subs, energies, sectors, propagation) if propagation.mode == "inelastic" and len(subs) > 1 else None
     results = []
-    for block in blocks:
-        sub = hamiltonian.block(block.channel_indices)
+    for k, (block, sub) in enumerate(zip(blocks, subs)):
         B, L = get_Bmat_BF_to_SF(sub.basis)
-        Y, Smat = solve_device(sub, energies, sectors, propagation, B, L)
+        if Ys is None:
+            Y, Smat = solve_device(sub, energies, sectors, propagation, B, L)
+        else:
+            Y, Smat = match_device(sub, Ys[k], energies, sectors, propagation, B, L)
         indices = np.asarray(block.channel_indices, dtype=np.int64)
         E_int = sub.basis.E_int
         open_idx = tuple(indices[np.flatnonzero(E_int < E)] for E in energies)

@@ -95,3 +95,74 @@ def test_blocks_match_oracle(gpu_device, approx, jmax, Jtot):
                 np.testing.assert_allclose(SBF @ SBF.conj().T, np.eye(SBF.shape[0]), atol=1e-9)
     print(f"{approx.value}: {len(blocks)} blocks, max |dS| = {worst:.3e}")
     assert worst <= TOL_S, worst
+
+
+def test_batched_block_sweep_matches_per_block_sweeps(gpu_device):
+    """Blocks of 7, 12 and 20 channels in one padded sweep (ticc_ldmd_propagate_blocks)
+    against each block swept alone."""
+    from pyticc_amd import _backend, _device
+    from pyticc_amd.propagation.logd import propagate_logD
+    from tests.helpers import random_W_batch
+    import torch
+
+    rng = np.random.default_rng(17)
+    sizes, nE, nS, Nmax = (7, 12, 20), 3, 9, 20
+    E = np.array([0.4, 0.9, 1.6])
+    mu = 2.0
+    hs = np.full(nS, 0.02)
+    pad = 2 * mu * E.max() + 1.0e4
+    Wall = np.zeros((len(sizes), 2 * nS + 1, Nmax, Nmax))
+    want = []
+    for b, n in enumerate(sizes):
+        Ws, Wm, We = random_W_batch(rng, nS, n, scale=15.0)
+        pts = np.empty((2 * nS + 1, n, n))  # consecutive radial points: sector s starts where s - 1 ends
+        pts[0], pts[1::2], pts[2::2] = Ws[0], Wm, We
+        Ws = pts[0:-1:2].copy()
+        Wall[b, :, :n, :n] = pts
+        Wall[b, :, np.arange(n, Nmax), np.arange(n, Nmax)] = pad
+        W0 = Ws[0][None] - 2 * mu * E[:, None, None] * np.eye(n)
+        Y0 = np.stack([np.diag(np.sqrt(np.abs(np.diag(w)))) for w in W0])
+        want.append(_device.to_host(propagate_logD(Y0, E, mu, hs, Ws, Wm, We)))
+    Wd = _device.to_device(Wall, gpu_device)
+    Y = torch.zeros((len(sizes) * nE, Nmax, Nmax), dtype=torch.float64, device=gpu_device)
+    for b, n in enumerate(sizes):
+        d = Wall[b, 0].diagonal()[None, :] - 2 * mu * E[:, None]
+        for e in range(nE):
+            Y[b * nE + e] = torch.diag(torch.as_tensor(np.sqrt(np.abs(d[e])), device=gpu_device))
+    n2 = Nmax * Nmax
+    base = _device.ptr(Wd)
+    _backend.call("ticc_ldmd_propagate_blocks", len(sizes), nE, Nmax, E.ctypes.data_as(_backend._dp), mu, nS,
+                  hs.ctypes.data_as(_backend._dp), base, base + 8 * n2, base + 16 * n2, 2 * n2, (2 * nS + 1) * n2,
+                  _device.ptr(Y), _device.stream_handle(gpu_device))
+    got = _device.to_host(Y)
+    for b, n in enumerate(sizes):
+        blk = got[b * nE : (b + 1) * nE]
+        np.testing.assert_allclose(blk[:, :n, :n], want[b], rtol=0, atol=1e-12 * np.abs(want[b]).max())
+        if n < Nmax:  # the padding channels stay decoupled
+            assert np.abs(blk[:, :n, n:]).max() == 0.0 and np.abs(blk[:, n:, :n]).max() == 0.0
+
+
+def test_batched_blocks_time(gpu_device):
+    """The CS blocks of the N = 66 atom + rotor system: one batched sweep against one sweep per block."""
+    import time
+    from pyticc_amd.propagation.runner import propagate, propagate_blocks
+
+    w = synthetic.atom_rotor(approx=ticc.Approx.CS, n_energy=20)
+    grid, ham, _ = oracle_problem(w)
+    subs = [ham.block(b.channel_indices) for b in build_k_blocks(ham)]
+    prop = ticc.Propagation(device="gpu")
+    propagate_blocks(subs, w.energies, grid.sectors[:40], prop)
+    torch_sync = __import__("torch").cuda.synchronize
+    t = time.perf_counter()
+    Yb = propagate_blocks(subs, w.energies, grid.sectors, prop)
+    torch_sync()
+    t_batched = time.perf_counter() - t
+    t = time.perf_counter()
+    Ys = [propagate(s, w.energies, grid.sectors, prop) for s in subs]
+    torch_sync()
+    t_seq = time.perf_counter() - t
+    print(f"CS, {len(subs)} blocks x 20 energies x {len(grid.sectors)} sectors: batched {t_batched:.2f} s, "
+          f"one sweep per block {t_seq:.2f} s")
+    for a, b in zip(Yb, Ys):
+        ref = b.cpu().numpy()
+        np.testing.assert_allclose(a.cpu().numpy(), ref, rtol=0, atol=1e-11 * np.abs(ref).max())
