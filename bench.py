@@ -25,6 +25,13 @@ work, on this host's cores.
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu-baseline]
        [--max-sectors M]   (first M sectors only: a short run for counter profiles;
                             the JSON line then names the reduced workload)
+       python bench.py --workload batching [--blocks 256] [--max-sectors 8]
+           the batching workload instead: 256 seeded two-rotor blocks of uneven
+           size (N log-uniform in 40-1200, each the N lowest channels of a
+           (J, parity) basis, J <= 8), 32 energies each, FP64; the blocks are
+           bucketed by the sweep autotuner's cost model and swept bucket by
+           bucket, against one sweep per block.  The per-sector cost does not
+           depend on R, so the first M sectors (from 4 bohr) give the rate.
 """
 
 from __future__ import annotations
@@ -106,6 +113,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-every", type=int, default=8, help="time every k-th batched inversion's fused launches")
     ap.add_argument("--max-sectors", type=int, default=0, help="propagate only the first M sectors (profiling runs)")
+    ap.add_argument("--workload", choices=("rotor_rotor", "batching"), default="rotor_rotor")
+    ap.add_argument("--blocks", type=int, default=256, help="batching workload: number of blocks")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (requested --gpus {args.gpus}); "
@@ -117,6 +126,8 @@ def main() -> int:
     import torch
 
     logging.basicConfig(level=logging.INFO, stream=sys.stderr, format="[pyticc_amd] %(message)s")
+    if args.workload == "batching":
+        return bench_batching(args)
     import pyticc_amd as ticc
     from pyticc_amd import _backend, synthetic
     from pyticc_amd.match.asymptotic import get_Bmat_BF_to_SF
@@ -219,6 +230,118 @@ def main() -> int:
         "sweep_tflops_credited": 4.0 * N**3 * nE * nS / (elapsed / args.steps) / 1e12,
         "max_unitarity_defect": defect,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_batching(args) -> int:
+    """Uneven blocks x 32 energies: autotuned buckets against one sweep per block."""
+    import torch
+
+    import pyticc_amd as ticc
+    from pyticc_amd import synthetic
+    from pyticc_amd.propagation.buckets import (
+        modelled_seconds,
+        plan_buckets,
+        plan_singletons,
+        propagate_buckets,
+        tune_sweep,
+    )
+    from pyticc_amd.scattering.hamiltonian import build_hamiltonian
+
+    n_sector = args.max_sectors if args.max_sectors > 0 else 8
+    n_energy = 32
+    t0 = time.perf_counter()
+    specs = synthetic.rotor_rotor_block_specs(seed=3, n_block=args.blocks)
+    h = 0.009
+    works = synthetic.rotor_rotor_blocks(specs, n_energy=n_energy, boundaries=(4.0, 4.0 + 2 * h * n_sector),
+                                         half_step=h)
+    first = next(iter(works.values()))
+    grid = ticc.prepare_potential(first.system, first.boundaries, first.half_steps, **first.quadrature)
+    full = {k: build_hamiltonian(w.system, grid) for k, w in works.items()}
+    hams = [full[(s.Jtot, s.parity)].truncate(s.n_channel) for s in specs]
+    sizes = np.array([s.n_channel for s in specs])
+    E = first.energies
+    sectors = grid.sectors
+    prop = ticc.Propagation(device="gpu", memory_mb=8192.0)
+    dev = torch.device("cuda", 0)
+    _log(f"setup {time.perf_counter() - t0:.1f} s: {len(hams)} blocks, N {sizes.min()}-{sizes.max()}, "
+         f"{len(works)} (J, parity) bases, {n_energy} energies, {len(sectors)} sectors")
+
+    t = time.perf_counter()
+    grid_pads = [n for n in (64, 128, 256, 384, 512, 768, 1024, 1216) if n <= max(64, sizes.max() + 31)]
+    cost = tune_sweep(grid_pads, batches=(32, 256), device="gpu")
+    _log(f"autotune {time.perf_counter() - t:.1f} s: " + ", ".join(
+        f"Np {n}: a {a * 1e3:.2f} ms + b {b * 1e6:.1f} us/matrix, walk {w[0]}"
+        for n, a, b, w in zip(cost.n_pads, cost.a, cost.b, cost.walks)))
+    plan = plan_buckets(sizes, n_energy, cost)
+    single = plan_singletons(sizes)
+    _log(f"plan: {len(plan)} buckets " + " ".join(f"{b.n_pad}x{len(b.blocks)}" for b in plan) +
+         f"; modelled {modelled_seconds(plan, n_energy, cost):.3f} s/sector against "
+         f"{modelled_seconds(single, n_energy, cost):.3f} per block")
+
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        propagate_buckets(hams, E, sectors[:2], prop, plan)
+        torch.cuda.synchronize()
+        _log(f"warmup {i}: {time.perf_counter() - t:.2f} s")
+
+    def run(p):
+        timings = []
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            timings = []
+            Ys = propagate_buckets(hams, E, sectors, prop, p, timings)
+        torch.cuda.synchronize()
+        return Ys, (time.perf_counter() - t) / args.steps, timings
+
+    Yb, t_b, tim_b = run(plan)
+    _log(f"bucketed: {t_b:.2f} s per step")
+    Ys, t_s, _ = run(single)
+    _log(f"per block: {t_s:.2f} s per step")
+    diff = max(float((a - b).abs().max() / b.abs().max()) for a, b in zip(Yb, Ys))
+
+    units = len(hams) * n_energy * len(sectors)
+    credited = 4.0 * float(np.sum(sizes.astype(np.float64) ** 3)) * n_energy * len(sectors)
+    buckets = []
+    for bt in tim_b:
+        nb = np.array([sizes[i] for i in bt.bucket.blocks], dtype=np.float64)
+        f = 4.0 * float(np.sum(nb**3)) * n_energy * len(sectors)
+        buckets.append({"n_pad": bt.bucket.n_pad, "n_blocks": len(nb), "n_min": int(nb.min()), "seconds": bt.seconds,
+                        "tflops_credited": f / bt.seconds / 1e12, "walk": list(bt.bucket.walk)})
+    line = {
+        "metric": "LogD energy.R-steps/s, uneven two-rotor blocks (FP64, bucketed batches)",
+        "value": units / t_b,
+        "unit": "energy.R-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_b,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Tang-Toennies rotor-rotor PES, rigid rotors)",
+        "config": {
+            "workload": f"{len(hams)} two-rotor blocks, N log-uniform 40-1200 (seed 3; the N lowest channels of a "
+            f"(J, parity) basis, J <= 8), {n_energy} energies 10.87-1000.87 cm^-1, the first {len(sectors)} "
+            "LDMD sectors from 4 bohr (h = 0.009); step = contraction + FP64 LDMD sweep of every block",
+            "n_block": len(hams),
+            "n_energy": n_energy,
+            "n_sector": len(sectors),
+            "precision": "fp64",
+        },
+        "per_block_value": units / t_s,
+        "speedup_vs_per_block": t_s / t_b,
+        "tflops_credited": credited / t_b / 1e12,
+        "frac_of_fp64_mfma_peak": credited / t_b / 1e12 / FP64_MFMA_PEAK_TFLOPS,
+        "max_rel_diff_bucketed_vs_per_block": diff,
+        "modelled_seconds": {"bucketed": modelled_seconds(plan, n_energy, cost) * len(sectors),
+                             "per_block": modelled_seconds(single, n_energy, cost) * len(sectors)},
+        "buckets": buckets,
+        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -35,7 +35,7 @@ extern "C" {
 #define TICC_ERR_UNSUPPORTED 5
 
 /* Largest channel count of one propagated block. */
-#define TICC_MAX_CHANNELS 1024
+#define TICC_MAX_CHANNELS 1536
 
 /* ABI version of the loaded library (TICC_ABI_VERSION). */
 int ticc_abi_version(void);
@@ -240,6 +240,16 @@ int ticc_mlp_adam(int rows, int cols, float* theta, const float* grad, float* m,
 
 /* BF16 copy and transpose of an FP32 rows x cols tensor. */
 int ticc_mlp_to_bf16(int rows, int cols, const float* x, void* bf, void* bfT, void* stream);
+
+/*
+ * Tile walk of the Gauss-Jordan update launches (no counterpart in the
+ * reference; set by the sweep autotuner per size bucket): the number of
+ * workgroups the column tiles of one rank-64 update are spread over, for the
+ * update that runs beside the next panel chain (default 512) and for the
+ * standalone updates (default 2048).  Applies to the calling thread's later
+ * sweeps and inversions; 0 restores a default.  Results do not depend on it.
+ */
+int ticc_set_sweep_walk(int fused_target, int update_target);
 
 /*
  * Instrumentation (no counterpart in the reference): live HIP-event timing of

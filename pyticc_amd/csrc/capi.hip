@@ -31,6 +31,7 @@ int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, con
              const double* Eint_host, const double* L_host, double* S_re, double* S_im, int* n_open,
              hipStream_t stream);
 int profile_begin(int every, int max_pairs);
+int gj_set_walk(int fused_target, int update_target);
 int profile_end(double* total_ms, int* count, double* total_bytes, double* total_flop);
 int bessel_eval(int n, const double* ell, const double* x, double* out, const double* nu, double* kout,
                 hipStream_t stream);
@@ -274,6 +275,10 @@ int ticc_mlp_adam(int rows, int cols, float* theta, const float* grad, float* m,
 int ticc_mlp_to_bf16(int rows, int cols, const float* x, void* bf, void* bfT, void* stream) {
   if (rows <= 0 || cols <= 0 || !x) return fail(TICC_ERR_VALUE, "mlp_to_bf16: invalid arguments");
   return ticc::mlp_to_bf16(rows, cols, x, bf, bfT, as_stream(stream));
+}
+
+int ticc_set_sweep_walk(int fused_target, int update_target) {
+  return ticc::gj_set_walk(fused_target, update_target);
 }
 
 int ticc_profile_update_begin(int sample_every, int max_samples) {

@@ -391,7 +391,7 @@ __device__ __forceinline__ double2 buffer_load_f64x2(__amdgpu_buffer_rsrc_t r, i
 // operands of GCH blocks are loaded together, so the gathers' latency is paid
 // twice per GCH blocks instead of twice per block.  sB stages the NB gathered
 // B rows.
-template <int THREADS>
+template <int THREADS, int MAXROWS>
 __device__ __forceinline__ void panel_gemm_body(const double* __restrict__ A, const double* __restrict__ C,
                                                 long long cb, int ldC, int colC, const int* __restrict__ gC,
                                                 const double* __restrict__ Bm, long long bb, int ldB, int colB,
@@ -399,7 +399,7 @@ __device__ __forceinline__ void panel_gemm_body(const double* __restrict__ A, co
                                                 double* __restrict__ Out, long long ostride, int b,
                                                 double (*sB)[NB + 1]) {
   constexpr int RB = THREADS / 4;                 // rows per round of all waves
-  constexpr int MAXIT = (MAX_NP + RB - 1) / RB;   // rounds at the largest Np
+  constexpr int MAXIT = (MAXROWS + RB - 1) / RB;  // rounds at the largest Np of the caller
   constexpr int GCH = 4;                          // rounds whose operands are in flight together
   static_assert(MAXIT % GCH == 0, "round count must be a multiple of the chunk");
   const int t = threadIdx.x;
@@ -495,6 +495,8 @@ struct PairJob {
 
 template <int RPT>
 __device__ __forceinline__ void gj_pair_chain_body(const PairJob& P, int b) {
+  // row bound of the panel GEMMs: 1024 up to two rows per thread, else MAX_NP
+  constexpr int GR = RPT <= 2 ? 1024 : MAX_NP;
   __shared__ double sG[NB][NB + 1];
   const int Np = P.Np, k0 = P.k0, k1 = P.k0 + NB;
   const long long pn = (long long)Np * NB;
@@ -507,13 +509,13 @@ __device__ __forceinline__ void gj_pair_chain_body(const PairJob& P, int b) {
   gj_panel_body<512, RPT>(P.Xc, P.xs, P.ldx, P.xcol, Np, k0, P.P1, P.s1, P.p1, P.perm, P.pinv, nullptr, 0, nullptr,
                           nullptr, 0, nullptr, nullptr, b);
   __syncthreads();
-  panel_gemm_body<512>(P.P1, P.Xc, P.xs, P.ldx, P.xcol + NB, P.s1, P.Xc, P.xs, P.ldx, P.xcol + NB, P.p1, k0, Np, P.Z,
+  panel_gemm_body<512, GR>(P.P1, P.Xc, P.xs, P.ldx, P.xcol + NB, P.s1, P.Xc, P.xs, P.ldx, P.xcol + NB, P.p1, k0, Np, P.Z,
                        0LL, b, sG);
   __syncthreads();
   gj_panel_body<512, RPT>(P.Z, pn, NB, 0, Np, k1, P.P2, P.s2, P.p2, P.perm, P.pinv, P.AT + (size_t)NB * Np, at_stride,
                           P.s1, P.p1, k0, P.src2, P.brow, b);
   __syncthreads();
-  panel_gemm_body<512>(P.P2, P.P1, pn, NB, 0, P.s2, P.P1, pn, NB, 0, P.p2, k1, Np, P.AT, at_stride, b, sG);
+  panel_gemm_body<512, GR>(P.P2, P.P1, pn, NB, 0, P.s2, P.P1, pn, NB, 0, P.p2, k1, Np, P.AT, at_stride, b, sG);
 }
 
 template <int RPT>
@@ -808,6 +810,14 @@ struct PanelMaps {  // optional outputs of a panel launch
   int* brow = nullptr;
 };
 
+// Tile walk of the update launches: the workgroup count a launch's column tiles
+// are spread over (fewer, longer walks amortise each band's prologue; more,
+// shorter ones fill the chip at small batch).  Per calling thread, 0 = default.
+constexpr int kWalkFused = 512;    // about two workgroups per CU beside the panel chains
+constexpr int kWalkUpdate = 2048;  // standalone update launches
+thread_local int t_walk_fused = 0;
+thread_local int t_walk_update = 0;
+
 int launch_panel(const double* X, long long bstride, int ld, int col0, int batch, int Np, int k0, double* pout,
                  int* src, int* piv, int* perm, int* pinv, const PanelMaps& m, hipStream_t stream) {
 #define TICC_PANEL_ARGS                                                                                          \
@@ -817,8 +827,10 @@ int launch_panel(const double* X, long long bstride, int ld, int col0, int batch
     hipLaunchKernelGGL((k_gj_panel<256, 1>), dim3(batch), dim3(256), 0, stream, TICC_PANEL_ARGS);
   else if (Np <= 512)
     hipLaunchKernelGGL((k_gj_panel<512, 1>), dim3(batch), dim3(512), 0, stream, TICC_PANEL_ARGS);
-  else
+  else if (Np <= 1024)
     hipLaunchKernelGGL((k_gj_panel<512, 2>), dim3(batch), dim3(512), 0, stream, TICC_PANEL_ARGS);
+  else
+    hipLaunchKernelGGL((k_gj_panel<512, 3>), dim3(batch), dim3(512), 0, stream, TICC_PANEL_ARGS);
 #undef TICC_PANEL_ARGS
   TICC_CHECK_LAUNCH();
   return TICC_OK;
@@ -832,8 +844,9 @@ int launch_update(const double* cur, double* out, long long ostride, int ldo, in
   constexpr int NBANDS = 1;
   const int units = ((Np + 63) / 64 + NBANDS - 1) / NBANDS;  // band groups of NBANDS bands
   const int span = T1 - T0;
-  // about 2048 / NBANDS workgroups: each walks span/ngroups tiles
-  int ngroups = (2048 / NBANDS + units * batch - 1) / (units * batch);
+  // about walk / NBANDS workgroups: each walks span/ngroups tiles
+  const int walk = t_walk_update > 0 ? t_walk_update : kWalkUpdate;
+  int ngroups = (walk / NBANDS + units * batch - 1) / (units * batch);
   if (ngroups > span) ngroups = span;
   if (ngroups < 1) ngroups = 1;
   hipLaunchKernelGGL((k_gj_update<KB, NBANDS>), dim3(units * ngroups, batch), dim3(256 * NBANDS), 0, stream, cur, out,
@@ -897,8 +910,10 @@ PairJob pair_job(const double* Xc, long long xs, int ldx, int xcol, int Np, int 
 int launch_pair_chain(const PairJob& P, int batch, hipStream_t stream) {
   if (P.Np <= 512)
     hipLaunchKernelGGL((k_gj_pair_chain<1>), dim3(batch), dim3(512), 0, stream, P);
-  else
+  else if (P.Np <= 1024)
     hipLaunchKernelGGL((k_gj_pair_chain<2>), dim3(batch), dim3(512), 0, stream, P);
+  else
+    hipLaunchKernelGGL((k_gj_pair_chain<3>), dim3(batch), dim3(512), 0, stream, P);
   TICC_CHECK_LAUNCH();
   return TICC_OK;
 }
@@ -922,7 +937,8 @@ int launch_fused(const PairJob& P, const double* X, double* Out, int batch, int 
   const int band_pairs = ((Np + 63) / 64 + 1) / 2;
   // about two workgroups per CU: longer tile walks amortise each band's
   // prologue (A fragments, first C and B tiles) better than more, shorter ones
-  int ngroups = (512 + band_pairs * batch - 1) / (band_pairs * batch);
+  const int walk = t_walk_fused > 0 ? t_walk_fused : kWalkFused;
+  int ngroups = (walk + band_pairs * batch - 1) / (band_pairs * batch);
   if (ngroups > T1 - T0) ngroups = T1 - T0;
   if (ngroups < 1) ngroups = 1;
   const int grid = batch + band_pairs * ngroups * batch;
@@ -941,8 +957,11 @@ int launch_fused(const PairJob& P, const double* X, double* Out, int batch, int 
   if (Np <= 512)
     hipLaunchKernelGGL((k_gj_fused<1>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
                        T0, T1, ngroups);
-  else
+  else if (Np <= 1024)
     hipLaunchKernelGGL((k_gj_fused<2>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
+                       T0, T1, ngroups);
+  else
+    hipLaunchKernelGGL((k_gj_fused<3>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
                        T0, T1, ngroups);
   TICC_CHECK_LAUNCH();
   if (sample) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
@@ -1007,6 +1026,14 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
     cur = buf[(pairs + 1) & 1];
   }
   *result = cur;
+  return TICC_OK;
+}
+
+int gj_set_walk(int fused_target, int update_target) {
+  if (fused_target < 0 || fused_target > 65536 || update_target < 0 || update_target > 65536)
+    return fail(TICC_ERR_VALUE, "tile walk targets must lie in [0, 65536], got %d and %d", fused_target, update_target);
+  t_walk_fused = fused_target;
+  t_walk_update = update_target;
   return TICC_OK;
 }
 

@@ -26,7 +26,7 @@ int aux_events(int n, hipEvent_t* out);
 // Panel width of the blocked Gauss-Jordan inversion and the padding quantum of
 // every internal square matrix (Np = round_up(N, GJ_NB)).
 constexpr int GJ_NB = 32;
-constexpr int MAX_NP = 1024;
+constexpr int MAX_NP = 1536;  // three panel rows per thread of a 512-thread workgroup
 // Scratch of gj_invert_raw per matrix: pout holds 9 Np x NB panels (two
 // transformed panels, the look-ahead columns of one and of two steps, and the
 // 2 NB transposed update coefficients of two pairs), src 4 Np row maps, piv 6

@@ -32,21 +32,24 @@ class VBasisBF:
     normalization: float = 1.0
     _device_cache: dict = field(default_factory=dict, compare=False, repr=False)
 
-    def restrict(self, channel_indices) -> "VBasisBF":
-        """The rows of the K subsets that make up `channel_indices` (whole K
-        subsets only), with channel numbers local to that list."""
+    def restrict(self, channel_indices, whole_k: bool = True) -> "VBasisBF":
+        """The rows of the channels in `channel_indices`, with channel numbers
+        local to that list.  With `whole_k` (CS / NNCC blocks) every helicity K
+        must be inside entirely or not at all; without it any subset is kept
+        (a basis truncated by threshold energy)."""
         local = {int(g): i for i, g in enumerate(channel_indices)}
         chan, Br, Bi = {}, {}, {} if self.B_imag is not None else None
         for K, idx in self.channel_indices.items():
             inside = [int(g) in local for g in idx]
             if not any(inside):
                 continue
-            if not all(inside):
+            if whole_k and not all(inside):
                 raise fail(ValueError, f"block channels must contain every channel of helicity K={K} or none")
-            chan[K] = [local[int(g)] for g in idx]
-            Br[K] = self.B_real[K]
+            rows = np.flatnonzero(inside)
+            chan[K] = [local[int(idx[r])] for r in rows]
+            Br[K] = self.B_real[K] if rows.size == len(idx) else np.ascontiguousarray(self.B_real[K][rows])
             if Bi is not None:
-                Bi[K] = self.B_imag[K]
+                Bi[K] = self.B_imag[K] if rows.size == len(idx) else np.ascontiguousarray(self.B_imag[K][rows])
         return VBasisBF(len(local), self.grid_shape, chan, Br, Bi, self.normalization)
 
     def device_arrays(self, device: torch.device):

@@ -1,0 +1,218 @@
+"""Bucketed LDMD sweep of many channel blocks of uneven size.
+
+Independent blocks -- (J, parity) problems, or the K blocks of several of them --
+swept at the same energies through the same radial grid.  The reference solves
+each (J, parity) problem on its own (src/pyticc/scattering/solver.py:57-178) and
+batches only the K blocks of one problem (src/pyticc/propagation/runner.py:326-374).
+
+Here the blocks are sorted by channel count and cut into contiguous buckets.
+Each bucket is swept as one (block x energy) batch at its largest member's size
+by ``propagate_blocks`` (``ticc_ldmd_propagate_blocks``); its smaller members
+are padded with decoupled closed channels, which leave their own channels'
+sweep unchanged.  A Gauss-Jordan launch costs about a(Np) + b(Np) * batch: the
+panel chain a(Np) is latency-bound and paid once per launch whatever the batch,
+while the rank-64 updates grow with the batch and with the padded size.  The
+cut trades the two and is chosen by dynamic programming on a cost model
+measured on the device (``tune_sweep``), which also picks the update's tile
+walk (``ticc_set_sweep_walk``) for each padded size.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from time import perf_counter
+
+import numpy as np
+import torch
+
+from pyticc_amd import _backend, _device
+from pyticc_amd._log import fail
+from pyticc_amd.energy import get_Etot
+from pyticc_amd.propagation.device import resolve_device
+from pyticc_amd.propagation.runner import propagate_blocks
+
+PAD_QUANTUM = 32  # every internal matrix is padded to a multiple of the panel width
+DEFAULT_WALK = (512, 2048)  # (beside the panel chain, standalone) update workgroup targets
+WALK_CANDIDATES = ((256, 1024), (512, 2048), (1024, 4096))
+
+
+def padded(n: int) -> int:
+    return -(-int(n) // PAD_QUANTUM) * PAD_QUANTUM
+
+
+@dataclass(frozen=True)
+class Bucket:
+    """Blocks swept together: their indices in the caller's list, the padded
+    channel count of the batch and the update tile walk used for it."""
+
+    n_pad: int
+    blocks: tuple
+    walk: tuple = DEFAULT_WALK
+
+
+@dataclass(frozen=True)
+class SweepCost:
+    """Seconds per sector of one batched sweep, t = a(Np) + b(Np) * batch,
+    piecewise linear in Np between the measured padded sizes, and the best
+    tile walk found at each of them."""
+
+    n_pads: tuple
+    a: tuple
+    b: tuple
+    walks: tuple = ()
+    samples: dict = field(default_factory=dict, compare=False)
+
+    def seconds(self, n_pad: int, batch: int) -> float:
+        x = np.asarray(self.n_pads, dtype=np.float64)
+        a = float(np.interp(n_pad, x, self.a))
+        b = float(np.interp(n_pad, x, self.b))
+        # beyond the largest measured size the per-matrix work grows as Np^3
+        if n_pad > x[-1]:
+            b = self.b[-1] * (n_pad / x[-1]) ** 3
+            a = self.a[-1] * (n_pad / x[-1])
+        return a + b * batch
+
+    def walk(self, n_pad: int) -> tuple:
+        if not self.walks:
+            return DEFAULT_WALK
+        i = int(np.argmin(np.abs(np.asarray(self.n_pads) - n_pad)))
+        return tuple(self.walks[i])
+
+
+def plan_buckets(sizes, n_energy: int, cost: SweepCost, max_batch_bytes: float = 64 * 2**30) -> tuple:
+    """Contiguous buckets of the size-sorted blocks minimising the modelled
+    sweep time sum_k t(Np_k, n_k * n_energy); a bucket's batch keeps its three
+    (batch, Np, Np) FP64 sweep buffers within max_batch_bytes."""
+    sizes = np.asarray(sizes, dtype=np.int64)
+    if sizes.ndim != 1 or sizes.size == 0 or np.any(sizes < 1):
+        raise fail(ValueError, "plan_buckets needs a non-empty list of positive channel counts")
+    if n_energy < 1:
+        raise fail(ValueError, f"n_energy must be positive, got {n_energy}")
+    order = np.argsort(sizes, kind="stable")
+    s = sizes[order]
+    n = s.size
+    best = np.full(n + 1, np.inf)
+    best[0] = 0.0
+    cut = np.zeros(n + 1, dtype=np.int64)
+    for i in range(1, n + 1):
+        npad = padded(s[i - 1])
+        for j in range(i - 1, -1, -1):  # bucket = sorted blocks j .. i-1
+            batch = (i - j) * n_energy
+            if j < i - 1 and 3.0 * 8.0 * batch * npad * npad > max_batch_bytes:
+                break
+            t = best[j] + cost.seconds(npad, batch)
+            if t <= best[i] * (1.0 + 1e-12):  # ties go to the larger bucket (fewer launches)
+                best[i], cut[i] = t, j
+    buckets = []
+    i = n
+    while i > 0:
+        j = int(cut[i])
+        npad = padded(s[i - 1])
+        buckets.append(Bucket(npad, tuple(int(k) for k in order[j:i]), cost.walk(npad)))
+        i = j
+    return tuple(reversed(buckets))
+
+
+def plan_singletons(sizes) -> tuple:
+    """One bucket per block, in the caller's order (the per-block launches)."""
+    return tuple(Bucket(padded(n), (i,)) for i, n in enumerate(sizes))
+
+
+def modelled_seconds(plan, n_energy: int, cost: SweepCost) -> float:
+    return float(sum(cost.seconds(b.n_pad, len(b.blocks) * n_energy) for b in plan))
+
+
+def _set_walk(walk) -> None:
+    _backend.call("ticc_set_sweep_walk", int(walk[0]), int(walk[1]))
+
+
+@dataclass
+class BucketTiming:
+    bucket: Bucket
+    seconds: float
+
+
+def propagate_buckets(hamiltonians, Etot, radial_sectors, config, plan, timings: list | None = None) -> list:
+    """Y_BF of every block (caller's order) after sweeping each bucket of
+    `plan` as one batch; `timings`, if given, receives each bucket's wall time
+    (device work included: every sweep call ends synchronised)."""
+    hams = list(hamiltonians)
+    seen = sorted(i for b in plan for i in b.blocks)
+    if seen != list(range(len(hams))):
+        raise fail(ValueError, "the bucket plan must hold every block exactly once")
+    energies = get_Etot(Etot)
+    out = [None] * len(hams)
+    try:
+        for bucket in plan:
+            members = [hams[i] for i in bucket.blocks]
+            if padded(max(h.basis.n_channel for h in members)) != bucket.n_pad:
+                raise fail(ValueError, f"bucket padded size {bucket.n_pad} does not fit its largest block")
+            _set_walk(bucket.walk)
+            t0 = perf_counter()
+            Ys = propagate_blocks(members, energies, radial_sectors, config)
+            if timings is not None:
+                timings.append(BucketTiming(bucket, perf_counter() - t0))
+            for i, Y in zip(bucket.blocks, Ys):
+                out[i] = Y
+    finally:
+        _set_walk((0, 0))
+    return out
+
+
+def _timed_sweep(Np: int, batch: int, n_sector: int, walk, dev: torch.device, seed: int) -> float:
+    """Wall seconds of one sweep call of `batch` random coupled matrices of size
+    Np over n_sector sectors (the call ends synchronised)."""
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    pts = 2 * n_sector + 1
+    C = torch.randn((Np, Np), generator=g, dtype=torch.float64) * (2.0 / np.sqrt(Np))
+    C = 0.5 * (C + C.T)
+    base = torch.diag(torch.linspace(50.0, 400.0, Np, dtype=torch.float64)) + C
+    W = base.to(dev).expand(pts, Np, Np).contiguous()
+    E = np.ascontiguousarray(np.linspace(1.0, 2.0, batch), dtype=np.float64)
+    Y = torch.eye(Np, dtype=torch.float64, device=dev).expand(batch, Np, Np).contiguous() * 3.0
+    hs = np.full(n_sector, 0.009)
+    n2 = Np * Np
+    ptr = _device.ptr(W)
+    _set_walk(walk)
+    torch.cuda.synchronize(dev)
+    t0 = perf_counter()
+    _backend.call("ticc_ldmd_propagate", batch, Np, E.ctypes.data_as(_backend._dp), 1.0, n_sector,
+                  hs.ctypes.data_as(_backend._dp), ptr, ptr + 8 * n2, ptr + 16 * n2, 2 * n2, _device.ptr(Y),
+                  _device.stream_handle(dev))
+    return perf_counter() - t0
+
+
+def tune_sweep(n_pads, batches=(32, 256), device="gpu", walks=WALK_CANDIDATES, sectors=(1, 3), seed=0) -> SweepCost:
+    """Measure t(Np, batch) per sector on the device at each padded size of
+    `n_pads` and each batch of `batches` (two or more), for each tile walk;
+    keep, per size, the walk fastest at the largest batch and fit a + b batch
+    through its samples (least squares, a and b clipped at zero)."""
+    n_pads = sorted({padded(n) for n in n_pads})
+    batches = sorted(set(int(b) for b in batches))
+    if len(batches) < 2 or not n_pads:
+        raise fail(ValueError, "tune_sweep needs at least one size and two batch counts")
+    dev = resolve_device(device).torch_device
+    s0, s1 = sorted(sectors)
+    samples = {}
+    A, B, Wk = [], [], []
+    try:
+        for Np in n_pads:
+            per = {}
+            for walk in walks:
+                ts = []
+                for batch in batches:
+                    _timed_sweep(Np, batch, 1, walk, dev, seed)  # warm-up: workspace, code objects
+                    t_lo = _timed_sweep(Np, batch, s0, walk, dev, seed)
+                    t_hi = _timed_sweep(Np, batch, s1, walk, dev, seed)
+                    ts.append(max((t_hi - t_lo) / (s1 - s0), 1e-7))
+                per[tuple(walk)] = ts
+                samples[(Np, tuple(walk))] = dict(zip(batches, ts))
+            walk = min(per, key=lambda w: per[w][-1])
+            ts = np.asarray(per[walk])
+            b, a = np.polyfit(np.asarray(batches, dtype=np.float64), ts, 1)
+            A.append(max(float(a), 0.0))
+            B.append(max(float(b), 1e-9))
+            Wk.append(walk)
+    finally:
+        _set_walk((0, 0))
+    return SweepCost(tuple(n_pads), tuple(A), tuple(B), tuple(Wk), samples)

@@ -63,6 +63,18 @@ class ScattHamiltonian:
         return ScattHamiltonian(basis, self.reduced_mass, self.vbasis.restrict(indices), self.potential_grid, self.approx,
                                 self.K_delta)
 
+    def truncate(self, n_channel: int) -> "ScattHamiltonian":
+        """The Hamiltonian on the n_channel lowest channels (the basis is sorted
+        by threshold energy, channel.py:569): W_base of the truncated basis is
+        the leading n_channel x n_channel block of the full one."""
+        n = int(n_channel)
+        if not 1 <= n <= self.basis.n_channel:
+            raise fail(ValueError, f"cannot truncate {self.basis.n_channel} channels to {n}")
+        indices = tuple(range(n))
+        basis = replace(self.basis, channels=tuple(self.basis[i] for i in indices))
+        return ScattHamiltonian(basis, self.reduced_mass, self.vbasis.restrict(indices, whole_k=False),
+                                self.potential_grid, self.approx, self.K_delta)
+
     def W_points(self, radial_points, device: torch.device, out: torch.Tensor | None = None) -> torch.Tensor:
         """Energy-independent W_base at the given grid points, (n_R, N, N) on the device."""
         U, Ei = self._device_consts(device)

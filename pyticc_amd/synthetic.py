@@ -135,3 +135,40 @@ def rotor_rotor(seed: int = 2, *, jmax_X: int = 4, jmax_Y: int = 9, Jtot: int = 
     E = np.linspace(*e_range_cm, n_energy) * CM2AU
     quad = {"n_theta_X": n_theta_X, "n_theta_Y": n_theta_Y, "n_phi": n_phi}
     return Workload("rotor_rotor", system, tuple(boundaries), (half_step,), quad, E)
+
+
+@dataclass(frozen=True)
+class BlockSpec:
+    """One block of the batching workload: the n_channel lowest channels of the
+    two-rotor (Jtot, parity) basis."""
+
+    Jtot: int
+    parity: int
+    n_channel: int
+
+
+def rotor_rotor_block_specs(seed: int = 3, *, n_block: int = 256, n_range=(40, 1200), Jmax: int = 8,
+                            **rotor_kw) -> tuple:
+    """Seeded blocks of uneven size: N log-uniform in n_range; each is the N lowest
+    channels (by threshold energy) of a (J, parity) basis of the two-rotor system
+    drawn uniformly among those with at least N channels (J <= Jmax)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = n_range
+    sizes = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), n_block))).astype(int)
+    counts = {(J, p): rotor_rotor(Jtot=J, parity=p, n_energy=1, **rotor_kw).system.basis.n_channel
+              for J in range(Jmax + 1) for p in (1, -1)}
+    specs = []
+    for n in sizes:
+        fits = sorted(k for k, c in counts.items() if c >= n)
+        if not fits:
+            raise ValueError(f"no (J, parity) basis with J <= {Jmax} holds {n} channels")
+        J, p = fits[int(rng.integers(len(fits)))]
+        specs.append(BlockSpec(J, p, int(n)))
+    return tuple(specs)
+
+
+def rotor_rotor_blocks(specs, *, n_energy: int = 32, **rotor_kw) -> dict:
+    """The two-rotor workload of every (J, parity) the block specs use (same PES,
+    radial grid, quadrature and energies), keyed by (J, parity)."""
+    keys = sorted({(s.Jtot, s.parity) for s in specs})
+    return {k: rotor_rotor(Jtot=k[0], parity=k[1], n_energy=n_energy, **rotor_kw) for k in keys}

@@ -31,7 +31,7 @@ def test_dgemm_batched_matches_numpy(gpu_device, m, n, k, batch):
 
 
 @pytest.mark.parametrize("N,batch", [(1, 1), (2, 3), (31, 2), (33, 4), (66, 5), (100, 3), (130, 2), (200, 3), (800, 2),
-                                     (1000, 1), (1024, 2)])
+                                     (1000, 1), (1024, 2), (1025, 2), (1200, 3), (1536, 1)])
 def test_invert_batched_matches_numpy(gpu_device, N, batch):
     rng = np.random.default_rng(N)
     A = rng.standard_normal((batch, N, N)) + np.eye(N) * 0.1
@@ -46,7 +46,7 @@ def test_invert_batched_matches_numpy(gpu_device, N, batch):
 
 def test_invert_above_maximum_size_is_rejected(gpu_device):
     with pytest.raises(NotImplementedError, match="exceeds"):
-        _backend.call("ticc_invert_batched", 1, 1025, None, None)
+        _backend.call("ticc_invert_batched", 1, 1537, None, None)
 
 
 def test_invert_requires_row_pivoting(gpu_device):

@@ -58,7 +58,7 @@ def test_sweep_matches_sequential_sector_updates(gpu_device):
     np.testing.assert_allclose(res, oldmd.propagate(Y0, E, mu, hs, Ws, Wm, We), rtol=1e-12, atol=1e-12)
 
 
-@pytest.mark.parametrize("N,n_E,n_S", [(3, 2, 5), (33, 3, 7), (66, 4, 40), (130, 2, 6), (1024, 2, 2)])
+@pytest.mark.parametrize("N,n_E,n_S", [(3, 2, 5), (33, 3, 7), (66, 4, 40), (130, 2, 6), (1024, 2, 2), (1200, 2, 2)])
 def test_sweep_matches_oracle_random_coupled(gpu_device, N, n_E, n_S):
     rng = np.random.default_rng(N * 7 + n_S)
     Ws, Wm, We = random_W_batch(rng, n_S, N, scale=20.0)
