@@ -4,10 +4,13 @@
 // G = [B_real | B_imag] (N_K x n_ang each):
 //   V_K(r) = norm (B_real diag(V_r) B_real^T + B_imag diag(V_r) B_imag^T)
 //   W(r)[a][b] = U[a][b] / R_r^2 + 2 mu V[a][b] (+ 2 mu E_int[a] on the diagonal)
-// Cross-K elements of V are zero (W there is U/R^2 only).  One wave computes
-// one 32 x 32 tile (a-tile <= b-tile) with FP64 MFMA, diag(V_r) applied to the
-// A operand as it is loaded, and writes both mirror images, so W is exactly
-// symmetric.
+// Cross-K elements of V are zero (W there is U/R^2 only).  One workgroup of
+// CW waves computes one 32 x 32 tile (a-tile <= b-tile) with FP64 MFMA: each
+// wave sums a contiguous CW-th of the angular grid, diag(V_r) applied to the A
+// operand as it is loaded; the partial tiles meet in LDS and the first wave
+// writes both mirror images, so W is exactly symmetric.  Splitting the
+// angular sum keeps a small block's few tiles from running as a handful of
+// long serial waves.
 #include "ticc_common.h"
 
 namespace ticc {
@@ -24,8 +27,10 @@ __global__ void k_w_init(const double* __restrict__ U, const double* __restrict_
   }
 }
 
+constexpr int CW = 4;  // waves per tile, each summing a CW-th of the angular grid
+
 // tiles[t] = (row offset o, block size nk, a-tile, b-tile)
-__global__ __launch_bounds__(64) void k_contract(const int4* __restrict__ tiles, const int* __restrict__ chan,
+__global__ __launch_bounds__(64 * CW) void k_contract(const int4* __restrict__ tiles, const int* __restrict__ chan,
                                                  const double* __restrict__ Br, const double* __restrict__ Bi,
                                                  int n_ang, double norm, const double* __restrict__ R,
                                                  const double* __restrict__ V, const double* __restrict__ U,
@@ -35,10 +40,15 @@ __global__ __launch_bounds__(64) void k_contract(const int4* __restrict__ tiles,
   const int r = blockIdx.y;
   const int o = tl.x, nk = tl.y;
   const int a0 = 32 * tl.z, b0 = 32 * tl.w;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
   const int lr = lane & 15;
   const int lq = lane >> 4;
   const double* Vr = V + (size_t)r * n_ang;
+  // this wave's k-steps of 4 angular points: [ks0, ks1)
+  const int n_steps = (n_ang + 3) / 4;
+  const int ks0 = (wave * n_steps) / CW;
+  const int ks1 = ((wave + 1) * n_steps) / CW;
 
   double4_t acc[2][2];
 #pragma unroll
@@ -61,8 +71,9 @@ __global__ __launch_bounds__(64) void k_contract(const int4* __restrict__ tiles,
       rows_a[q] = Bm + (size_t)(o + (va[q] ? la : 0)) * n_ang;
       rows_b[q] = Bm + (size_t)(o + (vb[q] ? lb : 0)) * n_ang;
     }
-    for (int g0 = 0; g0 < n_ang; g0 += 4) {
-      const int g = g0 + lq;
+#pragma unroll 4
+    for (int ks = ks0; ks < ks1; ++ks) {
+      const int g = 4 * ks + lq;
       const bool vg = g < n_ang;
       const double vv = vg ? Vr[g] : 0.0;
       double a[2], b[2];
@@ -78,6 +89,27 @@ __global__ __launch_bounds__(64) void k_contract(const int4* __restrict__ tiles,
           acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
     }
   }
+
+  // partial tiles of waves 1 .. CW-1 -> wave 0
+  __shared__ double red[CW - 1][16][64];
+  if (wave > 0) {
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[wave - 1][8 * bi + 4 * bj + q][lane] = acc[bi][bj][q];
+  }
+  __syncthreads();
+  if (wave > 0) return;
+#pragma unroll
+  for (int w = 0; w < CW - 1; ++w)
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[bi][bj][q] += red[w][8 * bi + 4 * bj + q][lane];
 
   const double RR = R[r];
   const double R2 = RR * RR;
@@ -146,7 +178,7 @@ int contract_w(int N, int n_ang, int n_block, const int* block_offsets, const in
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   delete[] host_tiles;
   if (e != hipSuccess) return fail(TICC_ERR_HIP, "tile upload failed: %s", hipGetErrorString(e));
-  hipLaunchKernelGGL(k_contract, dim3(n_tiles, nR), dim3(64), 0, stream, static_cast<const int4*>(ws), block_channels,
+  hipLaunchKernelGGL(k_contract, dim3(n_tiles, nR), dim3(64 * CW), 0, stream, static_cast<const int4*>(ws), block_channels,
                      Br, Bi, n_ang, norm, R, V, U, Eint, 2.0 * mu, N, W);
   TICC_CHECK_LAUNCH();
   return TICC_OK;
