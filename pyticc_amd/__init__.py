@@ -38,6 +38,8 @@ from pyticc_amd.propagation import Propagation
 from pyticc_amd.result import CoupledStatesResult, KBlockResult, ScatteringResult, Timing
 from pyticc_amd.scattering import PotentialGrid, ScattHamiltonian, prepare_potential, solve
 from pyticc_amd import report
+from pyticc_amd.input import run
+from pyticc_amd.basis.monomer import prepare_Diatom
 from pyticc_amd.system import Approx, ChannelSpec, MolInnerState, ScatteringType, ScattSystem, build_ScattSystem, element_mass_au, reduced_mass
 
 __all__ = [
@@ -46,5 +48,5 @@ __all__ = [
     "DiatomSpec", "EV2AU", "GHZ2AU", "HZ2AU", "K2AU", "KBlock", "KBlockResult", "KHZ2AU", "MHZ2AU", "MolInnerState", "PESWrapper", "PotentialGrid", "Propagation",
     "RovibBasis", "ScatteringResult", "ScatteringType", "ScattHamiltonian", "ScattSystem", "Timing",
     "build_ChannelBasis", "build_ScattSystem", "build_cs_blocks", "build_nncc_blocks", "element_mass_au", "get_Etot", "load_native_pes", "prepare_potential", "reduced_mass",
-    "report", "rigid_rotor", "solve",
+    "prepare_Diatom", "report", "rigid_rotor", "run", "solve",
 ]

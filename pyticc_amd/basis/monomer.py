@@ -117,3 +117,26 @@ def rigid_rotor(B_rot: float, jmax: int, r_e: float = 1.0) -> DiatomBasis:
     j = np.arange(jmax + 1)
     rovib = RovibBasis(grids=np.array([float(r_e)]), E_vj=(B_rot * j * (j + 1.0))[None, :], WF_vj=np.ones((1, 1, jmax + 1)))
     return DiatomBasis(rovib=rovib, energy_zero=0.0)
+
+
+def build_DiatomBasis(dvr, *, n_podvr: int, vmax: int, jmax: int, mass: float,
+                      energy_zero: float | None = None) -> DiatomBasis:
+    """Adiabatic diatom from a primitive sine DVR through the PODVR contraction
+    (basis/monomer/diatom.py:141-158); the energy zero defaults to E(v=0, j=0)."""
+    from pyticc_amd.basis.dvr import build_RovibPODVR
+
+    rovib = build_RovibPODVR(dvr, n_podvr, vmax, jmax, mass)
+    zero = float(rovib.E_vj[0, 0]) if energy_zero is None else float(energy_zero)
+    return DiatomBasis(rovib=rovib, energy_zero=zero)
+
+
+def prepare_Diatom(potential, *, r: tuple, n_dvr: int, n_podvr: int, vmax: int, jmax: int, mass: float,
+                   energy_zero: float | None = None) -> DiatomBasis:
+    """Diatomic monomer from its potential: sine DVR on r = (a, b), then PODVR
+    (basis/monomer/diatom.py:162-207)."""
+    from pyticc_amd.basis.dvr import build_SineDVR
+
+    if potential is None:
+        raise fail(ValueError, "Diatomic monomer preparation requires a monomer potential")
+    dvr = build_SineDVR(float(r[0]), float(r[1]), int(n_dvr), float(mass), potential)
+    return build_DiatomBasis(dvr, n_podvr=n_podvr, vmax=vmax, jmax=jmax, mass=mass, energy_zero=energy_zero)

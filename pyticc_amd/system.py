@@ -20,13 +20,21 @@ from pyticc_amd._log import fail
 from pyticc_amd.constants import AMU2AU
 
 ELEMENT_MASS_AMU: dict[str, float] = {
+    # most abundant isotope (standard atomic weight for He, Li, S), as the reference's table
     "H": 1.00782503223,
     "D": 2.01410177812,
+    "B": 11.00930536,
     "He": 4.002602,
-    "C": 12.0,
+    "Li": 6.938,
     "N": 14.00307400443,
     "O": 15.99491461957,
+    "C": 12.0,
+    "F": 18.99840316273,
+    "S": 32.06,
+    "Cl": 34.968852682,
     "Ar": 39.9623831237,
+    "K": 39.963998166,
+    "Rb": 86.909180531,
 }
 
 
