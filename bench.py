@@ -283,25 +283,29 @@ def bench_batching(args) -> int:
 
     for i in range(args.warmup):
         t = time.perf_counter()
-        propagate_buckets(hams, E, sectors[:2], prop, plan)
+        propagate_buckets(hams, E, sectors[:2], prop, plan, lanes=1)
+        propagate_buckets(hams, E, sectors[:2], prop, plan, lanes=2)  # the second lane's workspaces
         torch.cuda.synchronize()
         _log(f"warmup {i}: {time.perf_counter() - t:.2f} s")
 
-    def run(p):
+    def run(p, lanes=1):
         timings = []
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(args.steps):
             timings = []
-            Ys = propagate_buckets(hams, E, sectors, prop, p, timings)
+            Ys = propagate_buckets(hams, E, sectors, prop, p, timings, lanes=lanes)
         torch.cuda.synchronize()
         return Ys, (time.perf_counter() - t) / args.steps, timings
 
-    Yb, t_b, tim_b = run(plan)
-    _log(f"bucketed: {t_b:.2f} s per step")
+    Yb1, t_b1, tim_b = run(plan)
+    _log(f"bucketed, one lane: {t_b1:.2f} s per step")
+    Yb, t_b, _ = run(plan, lanes=2)
+    _log(f"bucketed, two lanes: {t_b:.2f} s per step")
     Ys, t_s, _ = run(single)
     _log(f"per block: {t_s:.2f} s per step")
     diff = max(float((a - b).abs().max() / b.abs().max()) for a, b in zip(Yb, Ys))
+    diff_lanes = max(float((a - b).abs().max()) for a, b in zip(Yb, Yb1))
 
     units = len(hams) * n_energy * len(sectors)
     credited = 4.0 * float(np.sum(sizes.astype(np.float64) ** 3)) * n_energy * len(sectors)
@@ -327,12 +331,16 @@ def bench_batching(args) -> int:
         "config": {
             "workload": f"{len(hams)} two-rotor blocks, N log-uniform 40-1200 (seed 3; the N lowest channels of a "
             f"(J, parity) basis, J <= 8), {n_energy} energies 10.87-1000.87 cm^-1, the first {len(sectors)} "
-            "LDMD sectors from 4 bohr (h = 0.009); step = contraction + FP64 LDMD sweep of every block",
+            "LDMD sectors from 4 bohr (h = 0.009); step = contraction + FP64 LDMD sweep of every block, "
+            "the autotuned buckets swept by two concurrent lanes",
             "n_block": len(hams),
             "n_energy": n_energy,
             "n_sector": len(sectors),
             "precision": "fp64",
         },
+        "lanes": 2,
+        "one_lane_value": units / t_b1,
+        "max_abs_diff_two_lanes_vs_one": diff_lanes,
         "per_block_value": units / t_s,
         "speedup_vs_per_block": t_s / t_b,
         "tflops_credited": credited / t_b / 1e12,

@@ -53,6 +53,19 @@ int ticc_set_device(int device);
 /* Free every library-owned workspace on the current device (synchronises it). */
 int ticc_release_workspace(void);
 
+/* Number of lanes of library state per device (see ticc_set_lane). */
+#define TICC_LANES 4
+
+/*
+ * Select the calling thread's lane (no counterpart in the reference; for
+ * concurrent sweeps from several host threads, as the bucketed batching
+ * driver runs them).  The library's workspaces, auxiliary streams and events
+ * are kept per device and lane: calls made on different lanes, from different
+ * threads on different streams, never share them and may run concurrently.
+ * Calls on one lane must not overlap.  Every thread starts on lane 0.
+ */
+int ticc_set_lane(int lane);
+
 /*
  * W(R) contraction + radial-matrix assembly, kernel A.
  * Replaces contract_device / _contract_block_device

@@ -31,6 +31,7 @@ _SIGNATURES = {
     "ticc_device_count": ([_ip], _c_int),
     "ticc_set_device": ([_c_int], _c_int),
     "ticc_release_workspace": ([], _c_int),
+    "ticc_set_lane": ([_c_int], _c_int),
     "ticc_contract_w": (
         [_c_int, _c_int, _c_int, _ip, _vp, _vp, _vp, _c_d, _c_int, _vp, _vp, _vp, _vp, _c_d, _vp, _vp],
         _c_int,

@@ -77,6 +77,8 @@ int ticc_set_device(int device) {
 
 int ticc_release_workspace(void) { return ticc::release_workspace(); }
 
+int ticc_set_lane(int lane) { return ticc::set_lane(lane); }
+
 int ticc_contract_w(int n_channel, int n_ang, int n_block, const int* block_offsets, const int* block_channels,
                     const double* B_real, const double* B_imag, double normalization, int n_R, const double* R,
                     const double* V, const double* U, const double* E_int, double reduced_mass, double* W,

@@ -18,6 +18,9 @@ int fail(int code, const char* fmt, ...);
 // from two host threads without sharing scratch memory.
 enum WorkspaceSlot { WS_SWEEP = 0, WS_CONTRACT = 1, WS_MATCH = 2, WS_INVERT = 3, WS_COUNT = 4 };
 int workspace(WorkspaceSlot slot, size_t bytes, void** out);
+// Select the calling thread's lane of library state (workspaces, auxiliary
+// streams and events), 0 <= lane < TICC_LANES; lane 0 by default.
+int set_lane(int lane);
 // Library-owned non-blocking streams (default priority) and events of the
 // current device, created once and reused; callers must not destroy them.
 int aux_streams(int n, hipStream_t* out);

@@ -16,11 +16,15 @@ struct Buffer {
   size_t size = 0;
 };
 constexpr int kMaxDevices = 64;
-Buffer g_ws[kMaxDevices][WS_COUNT];
+// Library state is kept per (device, lane): a host thread that selects its own
+// lane (set_lane) gets workspaces, auxiliary streams and events no other lane
+// touches, so sweeps on different lanes may run concurrently.
+Buffer g_ws[kMaxDevices][TICC_LANES][WS_COUNT];
 std::mutex g_ws_mutex;
 constexpr int kMaxAux = 16;
-hipStream_t g_streams[kMaxDevices][kMaxAux] = {};
-hipEvent_t g_events[kMaxDevices][kMaxAux * 4] = {};
+hipStream_t g_streams[kMaxDevices][TICC_LANES][kMaxAux] = {};
+hipEvent_t g_events[kMaxDevices][TICC_LANES][kMaxAux * 4] = {};
+thread_local int t_lane = 0;
 
 __global__ void k_pad_identity(const double* __restrict__ src, long long src_stride, int N, double* __restrict__ dst,
                                int Np) {
@@ -73,12 +77,18 @@ int fail(int code, const char* fmt, ...) {
 
 const char* last_error() { return g_error; }
 
+int set_lane(int lane) {
+  if (lane < 0 || lane >= TICC_LANES) return fail(TICC_ERR_VALUE, "lane must lie in [0, %d), got %d", TICC_LANES, lane);
+  t_lane = lane;
+  return TICC_OK;
+}
+
 int workspace(WorkspaceSlot slot, size_t bytes, void** out) {
   int dev = 0;
   TICC_CHECK_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= kMaxDevices) return fail(TICC_ERR_VALUE, "device index %d out of range", dev);
   std::lock_guard<std::mutex> lock(g_ws_mutex);
-  Buffer& buf = g_ws[dev][slot];
+  Buffer& buf = g_ws[dev][t_lane][slot];
   if (buf.size < bytes) {
     if (buf.ptr) {
       TICC_CHECK_HIP(hipDeviceSynchronize());
@@ -105,8 +115,9 @@ int aux_streams(int n, hipStream_t* out) {
   if (n > kMaxAux || dev >= kMaxDevices) return fail(TICC_ERR_VALUE, "too many auxiliary streams requested");
   std::lock_guard<std::mutex> lock(g_ws_mutex);
   for (int i = 0; i < n; ++i) {
-    if (!g_streams[dev][i]) TICC_CHECK_HIP(hipStreamCreateWithFlags(&g_streams[dev][i], hipStreamNonBlocking));
-    out[i] = g_streams[dev][i];
+    hipStream_t& st = g_streams[dev][t_lane][i];
+    if (!st) TICC_CHECK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    out[i] = st;
   }
   return TICC_OK;
 }
@@ -117,8 +128,9 @@ int aux_events(int n, hipEvent_t* out) {
   if (n > kMaxAux * 4 || dev >= kMaxDevices) return fail(TICC_ERR_VALUE, "too many auxiliary events requested");
   std::lock_guard<std::mutex> lock(g_ws_mutex);
   for (int i = 0; i < n; ++i) {
-    if (!g_events[dev][i]) TICC_CHECK_HIP(hipEventCreateWithFlags(&g_events[dev][i], hipEventDisableTiming));
-    out[i] = g_events[dev][i];
+    hipEvent_t& ev = g_events[dev][t_lane][i];
+    if (!ev) TICC_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    out[i] = ev;
   }
   return TICC_OK;
 }
@@ -128,12 +140,13 @@ int release_workspace() {
   TICC_CHECK_HIP(hipGetDevice(&dev));
   TICC_CHECK_HIP(hipDeviceSynchronize());
   std::lock_guard<std::mutex> lock(g_ws_mutex);
-  for (int s = 0; s < WS_COUNT; ++s) {
-    Buffer& buf = g_ws[dev][s];
-    if (buf.ptr) TICC_CHECK_HIP(hipFree(buf.ptr));
-    buf.ptr = nullptr;
-    buf.size = 0;
-  }
+  for (int l = 0; l < TICC_LANES; ++l)
+    for (int s = 0; s < WS_COUNT; ++s) {
+      Buffer& buf = g_ws[dev][l][s];
+      if (buf.ptr) TICC_CHECK_HIP(hipFree(buf.ptr));
+      buf.ptr = nullptr;
+      buf.size = 0;
+    }
   return TICC_OK;
 }
 

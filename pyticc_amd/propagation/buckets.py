@@ -132,30 +132,93 @@ class BucketTiming:
     seconds: float
 
 
-def propagate_buckets(hamiltonians, Etot, radial_sectors, config, plan, timings: list | None = None) -> list:
+def _sweep_bucket(bucket, hams, energies, radial_sectors, config, out, timings) -> None:
+    members = [hams[i] for i in bucket.blocks]
+    _set_walk(bucket.walk)
+    t0 = perf_counter()
+    Ys = propagate_blocks(members, energies, radial_sectors, config)
+    if timings is not None:
+        timings.append(BucketTiming(bucket, perf_counter() - t0))
+    for i, Y in zip(bucket.blocks, Ys):
+        out[i] = Y
+
+
+def _lane_worker(lane: int, take, hams, energies, radial_sectors, config, out, timings, dev, errors) -> None:
+    """Sweep the buckets `take()` hands out, on library lane `lane` and a stream of its own."""
+    try:
+        _backend.call("ticc_set_lane", lane)
+        stream = torch.cuda.Stream(device=dev)  # default priority
+        with torch.cuda.stream(stream):
+            while (bucket := take()) is not None:
+                _sweep_bucket(bucket, hams, energies, radial_sectors, config, out, timings)
+        stream.synchronize()
+    except BaseException as error:  # re-raised by the caller
+        errors.append(error)
+    finally:
+        _set_walk((0, 0))
+        _backend.call("ticc_set_lane", 0)
+
+
+def propagate_buckets(hamiltonians, Etot, radial_sectors, config, plan, timings: list | None = None,
+                      lanes: int = 2) -> list:
     """Y_BF of every block (caller's order) after sweeping each bucket of
     `plan` as one batch; `timings`, if given, receives each bucket's wall time
-    (device work included: every sweep call ends synchronised)."""
+    (device work included: every sweep call ends synchronised).
+
+    With lanes = 2 two host threads sweep buckets concurrently, each on its own
+    stream and library lane (ticc_set_lane): one takes the plan's buckets from
+    the most expensive down, the other from the cheapest up, so the latency-bound
+    small buckets run beside the update-bound large ones."""
     hams = list(hamiltonians)
     seen = sorted(i for b in plan for i in b.blocks)
     if seen != list(range(len(hams))):
         raise fail(ValueError, "the bucket plan must hold every block exactly once")
+    for bucket in plan:
+        if padded(max(hams[i].basis.n_channel for i in bucket.blocks)) != bucket.n_pad:
+            raise fail(ValueError, f"bucket padded size {bucket.n_pad} does not fit its largest block")
+    if lanes not in (1, 2):
+        raise fail(ValueError, f"lanes must be 1 or 2, got {lanes}")
     energies = get_Etot(Etot)
     out = [None] * len(hams)
-    try:
-        for bucket in plan:
-            members = [hams[i] for i in bucket.blocks]
-            if padded(max(h.basis.n_channel for h in members)) != bucket.n_pad:
-                raise fail(ValueError, f"bucket padded size {bucket.n_pad} does not fit its largest block")
-            _set_walk(bucket.walk)
-            t0 = perf_counter()
-            Ys = propagate_blocks(members, energies, radial_sectors, config)
-            if timings is not None:
-                timings.append(BucketTiming(bucket, perf_counter() - t0))
-            for i, Y in zip(bucket.blocks, Ys):
-                out[i] = Y
-    finally:
-        _set_walk((0, 0))
+    if lanes == 1:
+        try:
+            for bucket in plan:
+                _sweep_bucket(bucket, hams, energies, radial_sectors, config, out, timings)
+        finally:
+            _set_walk((0, 0))
+        return out
+
+    import threading
+
+    order = sorted(plan, key=lambda b: len(b.blocks) * b.n_pad**3)  # cheapest first
+    lock = threading.Lock()
+    ends = [0, len(order)]  # next cheapest, one past the next most expensive
+
+    def taker(from_top: bool):
+        def take():
+            with lock:
+                if ends[0] >= ends[1]:
+                    return None
+                if from_top:
+                    ends[1] -= 1
+                    return order[ends[1]]
+                ends[0] += 1
+                return order[ends[0] - 1]
+        return take
+
+    dev = resolve_device(config.device).torch_device
+    errors: list = []
+    torch.cuda.current_stream(dev).synchronize()  # inputs made on the caller's stream are ready
+    workers = [threading.Thread(target=_lane_worker, args=(lane, taker(lane == 0), hams, energies, radial_sectors,
+                                                           config, out, timings, dev, errors))
+               for lane in (0, 1)]
+    # lane 0 runs in a new thread too, so the caller's own lane state is untouched
+    for w in workers:
+        w.start()
+    for w in workers:
+        w.join()
+    if errors:
+        raise errors[0]
     return out
 
 

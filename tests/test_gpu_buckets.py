@@ -52,6 +52,25 @@ def test_bucketed_sweep_matches_each_block_and_the_oracle(gpu_device):
         np.testing.assert_allclose(a, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
 
 
+def test_two_lanes_equal_one_lane(gpu_device):
+    """Buckets swept concurrently by two host threads on two library lanes and
+    streams give the same results, bit for bit, as one after another."""
+    sizes = (20, 47, 33, 55, 61, 90, 100)
+    subs, E, grid = _blocks(sizes, Js=(2, 1, 2, 1, 2, 2, 2))
+    prop = ticc.Propagation(device="gpu")
+    plan = (Bucket(padded(47), (0, 2, 1)), Bucket(padded(61), (3, 4)), Bucket(96, (5,)), Bucket(padded(100), (6,)))
+    one = propagate_buckets(subs, E, grid.sectors, prop, plan, lanes=1)
+    timings = []
+    two = propagate_buckets(subs, E, grid.sectors, prop, plan, timings=timings, lanes=2)
+    assert len(timings) == len(plan)
+    for a, b in zip(one, two):
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+    with pytest.raises(ValueError, match="lanes"):
+        propagate_buckets(subs, E, grid.sectors, prop, plan, lanes=3)
+    with pytest.raises(ValueError):
+        _backend.call("ticc_set_lane", 4)
+
+
 def test_tile_walk_does_not_change_results(gpu_device):
     subs, E, grid = _blocks((90,), Js=(2,))
     prop = ticc.Propagation(device="gpu")
