@@ -203,16 +203,22 @@ __device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long
     m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x121, 0xF, 0xF, false));  // row_ror:1
     const unsigned M = max(max((unsigned)__builtin_amdgcn_readlane((int)m, 0), (unsigned)__builtin_amdgcn_readlane((int)m, 16)),
                            max((unsigned)__builtin_amdgcn_readlane((int)m, 32), (unsigned)__builtin_amdgcn_readlane((int)m, 48)));
+    // A lane has a candidate whenever cq >= 0, even when |a| is zero or
+    // subnormal (its key then lies below 2^32 and khi is 0): a column of exact
+    // zeros below the diagonal still yields a pivot (value 0, so the inverse is
+    // non-finite, as LAPACK reports a singular matrix) and never an empty
+    // selection.
     int wl = -1;
-    if (M != 0u) {
-      const unsigned long long tied = __ballot(khi == M);
+    if (__ballot(cq >= 0) != 0ull) {
+      const bool top = cq >= 0 && khi == M;
+      const unsigned long long tied = __ballot(top);
       if (__popcll(tied) == 1) {
         wl = __ffsll((long long)tied) - 1;
       } else {
-        double wv = (khi == M) ? cv : -1.0;
-        int wp = (khi == M) ? cp : 0x7fffffff;
+        double wv = top ? cv : -1.0;
+        int wp = top ? cp : 0x7fffffff;
         wave_argmax_dpp(wv, wp);
-        wl = __ffsll((long long)__ballot(khi == M && cv == wv && cp == wp)) - 1;
+        wl = __ffsll((long long)__ballot(top && cv == wv && cp == wp)) - 1;
       }
     }
     // The wave's winning lane publishes its row (each of a thread's rows has its

@@ -49,6 +49,25 @@ def test_invert_above_maximum_size_is_rejected(gpu_device):
         _backend.call("ticc_invert_batched", 1, 1537, None, None)
 
 
+@pytest.mark.parametrize("N", [40, 64, 800])
+def test_invert_singular_zero_column_gives_nonfinite_not_a_fault(gpu_device, N):
+    """A column of exact zeros leaves every pivot candidate at |a| = 0: the
+    inversion must still pick a pivot (value 0) and return non-finite values, as
+    LAPACK reports a singular matrix, while the regular matrix of the same batch
+    is inverted exactly as alone."""
+    rng = np.random.default_rng(N + 1)
+    A = rng.standard_normal((3, N, N)) + np.eye(N) * 0.1
+    A[0] = 0.0
+    A[1][:, N // 3] = 0.0
+    d = _device.to_device(A, gpu_device)
+    _backend.call("ticc_invert_batched", 3, N, d.data_ptr(), _stream(gpu_device))
+    inv = _device.to_host(d)
+    assert not np.all(np.isfinite(inv[0])) and not np.all(np.isfinite(inv[1]))
+    alone = _device.to_device(A[2:], gpu_device)
+    _backend.call("ticc_invert_batched", 1, N, alone.data_ptr(), _stream(gpu_device))
+    np.testing.assert_array_equal(inv[2], _device.to_host(alone)[0])
+
+
 def test_invert_requires_row_pivoting(gpu_device):
     # zero leading diagonal: Gauss-Jordan without pivoting would divide by zero
     A = np.array([[[0.0, 1.0, 2.0], [1.0, 0.0, 3.0], [4.0, 5.0, 0.0]]])
