@@ -27,9 +27,9 @@ __global__ void k_w_init(const double* __restrict__ U, const double* __restrict_
   }
 }
 
-constexpr int CW = 4;  // waves per tile, each summing a CW-th of the angular grid
-
-// tiles[t] = (row offset o, block size nk, a-tile, b-tile)
+// tiles[t] = (row offset o, block size nk, a-tile, b-tile); CW waves per tile,
+// each summing a CW-th of the angular grid
+template <int CW>
 __global__ __launch_bounds__(64 * CW) void k_contract(const int4* __restrict__ tiles, const int* __restrict__ chan,
                                                  const double* __restrict__ Br, const double* __restrict__ Bi,
                                                  int n_ang, double norm, const double* __restrict__ R,
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(64 * CW) void k_contract(const int4* __restrict__ t
       rows_a[q] = Bm + (size_t)(o + (va[q] ? la : 0)) * n_ang;
       rows_b[q] = Bm + (size_t)(o + (vb[q] ? lb : 0)) * n_ang;
     }
-#pragma unroll 4
+#pragma unroll(CW > 1 ? 4 : 1)
     for (int ks = ks0; ks < ks1; ++ks) {
       const int g = 4 * ks + lq;
       const bool vg = g < n_ang;
@@ -91,8 +91,8 @@ __global__ __launch_bounds__(64 * CW) void k_contract(const int4* __restrict__ t
   }
 
   // partial tiles of waves 1 .. CW-1 -> wave 0
-  __shared__ double red[CW - 1][16][64];
-  if (wave > 0) {
+  __shared__ double red[CW > 1 ? CW - 1 : 1][16][64];
+  if (CW > 1 && wave > 0) {
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(64 * CW) void k_contract(const int4* __restrict__ t
 #pragma unroll
         for (int q = 0; q < 4; ++q) red[wave - 1][8 * bi + 4 * bj + q][lane] = acc[bi][bj][q];
   }
-  __syncthreads();
+  if (CW > 1) __syncthreads();
   if (wave > 0) return;
 #pragma unroll
   for (int w = 0; w < CW - 1; ++w)
@@ -178,8 +178,15 @@ int contract_w(int N, int n_ang, int n_block, const int* block_offsets, const in
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   delete[] host_tiles;
   if (e != hipSuccess) return fail(TICC_ERR_HIP, "tile upload failed: %s", hipGetErrorString(e));
-  hipLaunchKernelGGL(k_contract, dim3(n_tiles, nR), dim3(64 * CW), 0, stream, static_cast<const int4*>(ws), block_channels,
-                     Br, Bi, n_ang, norm, R, V, U, Eint, 2.0 * mu, N, W);
+  // Many tiles fill the chip with one wave each; few (a small block's) split
+  // each tile's angular sum over four waves instead.
+  const long long waves = (long long)n_tiles * nR;
+  if (waves >= 4096)
+    hipLaunchKernelGGL(k_contract<1>, dim3(n_tiles, nR), dim3(64), 0, stream, static_cast<const int4*>(ws),
+                       block_channels, Br, Bi, n_ang, norm, R, V, U, Eint, 2.0 * mu, N, W);
+  else
+    hipLaunchKernelGGL(k_contract<4>, dim3(n_tiles, nR), dim3(256), 0, stream, static_cast<const int4*>(ws),
+                       block_channels, Br, Bi, n_ang, norm, R, V, U, Eint, 2.0 * mu, N, W);
   TICC_CHECK_LAUNCH();
   return TICC_OK;
 }
