@@ -118,7 +118,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (requested --gpus {args.gpus}); "
-              "the energy/(J,p)-block sharding across GPUs is described in DESIGN.md", file=sys.stderr)
+              "energies and (J, parity) blocks are independent and would be split across GPUs", file=sys.stderr)
         return 2
 
     import logging
