@@ -84,18 +84,18 @@ def test_energy_at_threshold_raises(gpu_device):
         get_Smat(np.eye(2)[None], 5.0, [0.1], 1.0, [0.1, 0.0], [0.0, 1.0])
 
 
-def test_matching_matches_oracle_random_batch(gpu_device):
+@pytest.mark.parametrize("N,E,tol", [(70, (0.05, 0.5, 1.0, 1.5, 2.5, 0.77), 1e-11), (1100, (0.5, 1.5), 1e-10)])
+def test_matching_matches_oracle_random_batch(gpu_device, N, E, tol):
     rng = np.random.default_rng(11)
-    N, nE = 70, 6
-    A = rng.standard_normal((nE, N, N))
+    E = np.asarray(E)
+    A = rng.standard_normal((E.size, N, N))
     Y = 0.5 * (A + np.swapaxes(A, 1, 2)) + np.diag(rng.uniform(1.0, 5.0, N))
     E_int = np.sort(rng.uniform(0.0, 2.0, N))
     L = rng.integers(0, 12, N) + rng.choice([0.0, 0.3], N)
-    E = np.array([0.05, 0.5, 1.0, 1.5, 2.5, 0.77])
     B = np.linalg.qr(rng.standard_normal((N, N)))[0]
     S_re, S_im, n_open = get_Smat_device(Y, B, 12.0, E, 3.0, E_int, L, gpu_device)
     got = unpack_smat(S_re, S_im, n_open)
     ref = omatch.get_smat(omatch.transform_bf_to_sf(Y, B), 12.0, E, 3.0, E_int, L)
     for g, r in zip(got, ref):
         assert g.shape == r.shape
-        np.testing.assert_allclose(g, r, rtol=0, atol=1e-11)
+        np.testing.assert_allclose(g, r, rtol=0, atol=tol)
