@@ -100,6 +100,18 @@ def propagate_blocks(hamiltonians, Etot, radial_sectors, config) -> list:
     size = window_size(len(sectors), hamiltonians[0].potential_grid_size, Nmax * int(np.ceil(np.sqrt(nb))),
                        config.memory_mb)
     hs_all = np.asarray([s.radial_half_step for s in sectors])
+    # Blocks truncated from one Hamiltonian share its W: the largest of them is
+    # contracted and the others take its leading blocks (the same numbers).
+    source = list(range(nb))
+    largest: dict = {}
+    for b, ham in enumerate(hamiltonians):
+        root = getattr(ham, "truncated_from", None)
+        if root is not None and (id(root) not in largest or sizes[b] > sizes[largest[id(root)]]):
+            largest[id(root)] = b
+    for b, ham in enumerate(hamiltonians):
+        root = getattr(ham, "truncated_from", None)
+        if root is not None:
+            source[b] = largest[id(root)]
     Y = None
     W_prev = None
     for w0 in range(0, len(sectors), size):
@@ -108,6 +120,8 @@ def propagate_blocks(hamiltonians, Etot, radial_sectors, config) -> list:
         Wall = torch.zeros((nb, pts.size, Nmax, Nmax), dtype=torch.float64, device=dev)
         reuse = W_prev is not None and np.isclose(pts[0], W_prev[0], rtol=0.0, atol=1.0e-12)
         for b, (ham, n) in enumerate(zip(hamiltonians, sizes)):
+            if source[b] != b:
+                continue
             Wb = torch.empty((pts.size, n, n), dtype=torch.float64, device=dev)
             if reuse:
                 Wb[0].copy_(W_prev[1][b, :n, :n])
@@ -115,9 +129,11 @@ def propagate_blocks(hamiltonians, Etot, radial_sectors, config) -> list:
             else:
                 ham.W_points(pts, dev, out=Wb)
             Wall[b, :, :n, :n] = Wb
+        for b, n in enumerate(sizes):
+            if source[b] != b:
+                Wall[b, :, :n, :n] = Wall[source[b], :, :n, :n]
             if n < Nmax:
-                idx = torch.arange(n, Nmax, device=dev)
-                Wall[b, :, idx, idx] = pad
+                Wall[b, :, n:, n:].diagonal(dim1=-2, dim2=-1).fill_(pad)
         if Y is None:
             Y = torch.cat([initial_logD_energies(Wall[b, 0], E, mu, dev) for b in range(nb)])
         n2 = Nmax * Nmax

@@ -30,6 +30,8 @@ class ScattHamiltonian:
     approx: Approx = Approx.EXACT
     K_delta: int = 1
     _cache: dict = field(default_factory=dict, compare=False, repr=False)
+    # set by truncate(): the untruncated Hamiltonian this one is the leading block of
+    truncated_from: object = field(default=None, compare=False, repr=False)
 
     @property
     def E_int(self) -> np.ndarray:
@@ -72,8 +74,9 @@ class ScattHamiltonian:
             raise fail(ValueError, f"cannot truncate {self.basis.n_channel} channels to {n}")
         indices = tuple(range(n))
         basis = replace(self.basis, channels=tuple(self.basis[i] for i in indices))
+        root = self if self.truncated_from is None else self.truncated_from
         return ScattHamiltonian(basis, self.reduced_mass, self.vbasis.restrict(indices, whole_k=False),
-                                self.potential_grid, self.approx, self.K_delta)
+                                self.potential_grid, self.approx, self.K_delta, truncated_from=root)
 
     def W_points(self, radial_points, device: torch.device, out: torch.Tensor | None = None) -> torch.Tensor:
         """Energy-independent W_base at the given grid points, (n_R, N, N) on the device."""
