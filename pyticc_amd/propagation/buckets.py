@@ -168,7 +168,9 @@ def propagate_buckets(hamiltonians, Etot, radial_sectors, config, plan, timings:
     With lanes = 2 two host threads sweep buckets concurrently, each on its own
     stream and library lane (ticc_set_lane): one takes the plan's buckets from
     the most expensive down, the other from the cheapest up, so the latency-bound
-    small buckets run beside the update-bound large ones."""
+    small buckets run beside the update-bound large ones.  The two threads use
+    lanes 0 and 1 of the device, so this function must not itself run on
+    several host threads at once."""
     hams = list(hamiltonians)
     seen = sorted(i for b in plan for i in b.blocks)
     if seen != list(range(len(hams))):

@@ -115,3 +115,24 @@ def test_truncated_hamiltonian_is_the_leading_block():
         ham.truncate(N + 1)
     with pytest.raises(ValueError):
         ham.block(range(n))  # CS / NNCC blocks still need whole K subsets
+
+
+class _StubHam:
+    """Just what propagate_buckets validates before any device work."""
+
+    def __init__(self, n):
+        self.basis = type("B", (), {"n_channel": n})()
+
+
+def test_propagate_buckets_validates_the_plan_before_device_work():
+    from pyticc_amd.propagation.buckets import propagate_buckets
+
+    hams = [_StubHam(20), _StubHam(40)]
+    with pytest.raises(ValueError, match="exactly once"):
+        propagate_buckets(hams, [0.1], (), None, (Bucket(32, (0,)),))
+    with pytest.raises(ValueError, match="exactly once"):
+        propagate_buckets(hams, [0.1], (), None, (Bucket(64, (0, 1)), Bucket(32, (0,))))
+    with pytest.raises(ValueError, match="does not fit"):
+        propagate_buckets(hams, [0.1], (), None, (Bucket(32, (0, 1)),))
+    with pytest.raises(ValueError, match="lanes"):
+        propagate_buckets(hams, [0.1], (), None, (Bucket(64, (0, 1)),), lanes=3)
