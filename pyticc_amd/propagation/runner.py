@@ -37,9 +37,28 @@ def window_size(n_sector: int, n_grid: int, n_channel: int, memory_mb: float) ->
     return max(1, min(n_sector, _MAX_WINDOW, est))
 
 
+def _window_W(hamiltonian, window, prev, N: int, dev: torch.device):
+    """W_base at a window's points on the current stream; `prev` = (R, W) of the
+    previous window's last point, reused when the windows share it (runner.py:258-263)."""
+    pts = window_points(window)
+    Wpts = torch.empty((pts.size, N, N), dtype=torch.float64, device=dev)
+    if prev is not None and np.isclose(pts[0], prev[0], rtol=0.0, atol=1.0e-12):
+        Wpts[0].copy_(prev[1])
+        hamiltonian.W_points(pts[1:], dev, out=Wpts[1:])
+    else:
+        hamiltonian.W_points(pts, dev, out=Wpts)
+    return pts, Wpts
+
+
 def propagate(hamiltonian, Etot, radial_sectors, config) -> torch.Tensor:
     """Y_BF at the last sector end for every energy on the device: (n_E, N, N) float64, or for
-    mode="capture" the real forms (n_E, 2N, 2N) of the complex log derivatives."""
+    mode="capture" the real forms (n_E, 2N, 2N) of the complex log derivatives.
+
+    As the reference's prefetch thread does (runner.py:92-108), the next window's
+    W_base is contracted on a second host thread and stream while the current
+    window is swept; the sweep's stream waits for it by an event."""
+    from concurrent.futures import ThreadPoolExecutor
+
     energies = get_Etot(Etot)
     capture = config.mode == "capture"
     sectors = tuple(radial_sectors)
@@ -51,32 +70,44 @@ def propagate(hamiltonian, Etot, radial_sectors, config) -> torch.Tensor:
     mu = hamiltonian.reduced_mass
     size = window_size(len(sectors), hamiltonian.potential_grid_size, N, config.memory_mb)
     hs_all = np.asarray([s.radial_half_step for s in sectors])
+    starts = list(range(0, len(sectors), size))
     progress_every = max(1, ceil(len(sectors) / 10))
     next_progress = progress_every
     started = perf_counter()
+    main = torch.cuda.current_stream(dev)
+    side = torch.cuda.Stream(device=dev)  # default priority
+    prev = [None]
+
+    def contract(w0):  # on the prefetch thread
+        with torch.cuda.device(dev), torch.cuda.stream(side):
+            pts, W = _window_W(hamiltonian, sectors[w0 : w0 + size], prev[0], N, dev)
+            prev[0] = (float(pts[-1]), W[-1])
+            ready = torch.cuda.Event()
+            ready.record(side)
+        return pts, W, ready
+
     Y = None
-    W_prev = None
     done = 0
-    for w0 in range(0, len(sectors), size):
-        window = sectors[w0 : w0 + size]
-        pts = window_points(window)
-        Wpts = torch.empty((pts.size, N, N), dtype=torch.float64, device=dev)
-        if W_prev is not None and np.isclose(pts[0], W_prev[0], rtol=0.0, atol=1.0e-12):
-            Wpts[0].copy_(W_prev[1])
-            hamiltonian.W_points(pts[1:], dev, out=Wpts[1:])
-        else:
-            hamiltonian.W_points(pts, dev, out=Wpts)
-        if Y is None:
-            init = initial_logD_capture_energies if capture else initial_logD_energies
-            Y = init(Wpts[0], energies, mu, dev)
-        sweep = propagate_points_capture if capture else propagate_points
-        sweep(Y, energies, mu, hs_all[w0 : w0 + len(window)], Wpts, dev)
-        W_prev = (float(pts[-1]), Wpts[-1].clone())
-        done += len(window)
-        if config.print_verbose and (done == len(sectors) or done >= next_progress):
-            logger.info(f"Propagation: {done}/{len(sectors)} sectors, R={pts[-1]:.6f} bohr, wall={perf_counter() - started:.3f} s")
-            while next_progress <= done:
-                next_progress += progress_every
+    side.wait_stream(main)  # work the caller queued before this call
+    with ThreadPoolExecutor(max_workers=1, thread_name_prefix="ticc-prefetch") as pool:
+        pending = pool.submit(contract, starts[0])
+        for k, w0 in enumerate(starts):
+            pts, Wpts, ready = pending.result()
+            if k + 1 < len(starts):
+                pending = pool.submit(contract, starts[k + 1])
+            main.wait_event(ready)
+            Wpts.record_stream(main)
+            window = sectors[w0 : w0 + size]
+            if Y is None:
+                init = initial_logD_capture_energies if capture else initial_logD_energies
+                Y = init(Wpts[0], energies, mu, dev)
+            sweep = propagate_points_capture if capture else propagate_points
+            sweep(Y, energies, mu, hs_all[w0 : w0 + len(window)], Wpts, dev)
+            done += len(window)
+            if config.print_verbose and (done == len(sectors) or done >= next_progress):
+                logger.info(f"Propagation: {done}/{len(sectors)} sectors, R={pts[-1]:.6f} bohr, wall={perf_counter() - started:.3f} s")
+                while next_progress <= done:
+                    next_progress += progress_every
     return Y
 
 
