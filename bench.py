@@ -53,42 +53,48 @@ def _log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(workload, grid, min_seconds: float = 10.0, max_sectors: int = 64) -> dict:
-    """Time the oracle LDMD (contraction + sweep) on the host for a bounded sample."""
+def cpu_baseline(workload, grid, min_seconds: float = 10.0, max_sectors: int = 64, repeats: int = 3) -> dict:
+    """Time the oracle LDMD (contraction + sweep) on the host for a bounded sample:
+    the first sectors from the initial condition, as many as take about
+    min_seconds / repeats, run `repeats` times; the median rate is reported."""
     import threadpoolctl
 
-    from oracle import contraction as oc
     from oracle import ldmd
+    from oracle.pipeline import w_points
     from tests.helpers import oracle_problem
 
     _, _, prob = oracle_problem(workload)
     E = workload.energies
     hs = np.asarray([s.radial_half_step for s in grid.sectors])
-    from oracle.pipeline import w_points
-
     info = threadpoolctl.threadpool_info()
     threads = max((d.get("num_threads", 1) for d in info), default=1)
-    t0 = time.perf_counter()
-    W_prev = w_points(prob, np.array([0]))[0]
-    Y = ldmd.initialize_inelastic(W_prev[None] - 2 * prob.mu * E[:, None, None] * np.eye(prob.n_channel))
     shift = 2 * prob.mu * E
-    n = 0
-    while n < max_sectors and (time.perf_counter() - t0 < min_seconds or n < 2):
-        Wm, We = w_points(prob, np.array([2 * n + 1, 2 * n + 2]))
-        ref, Qs, Qm, Qe = ldmd.correction_matrices(hs[n], W_prev, Wm, We)
-        for e in range(E.size):
-            Y[e] = ldmd.sector_step(Y[e], hs[n], ref - shift[e], Qs, Qm, Qe)
-        W_prev = We
-        n += 1
-    dt = time.perf_counter() - t0
-    del oc
+
+    def run(n_max, budget):
+        t0 = time.perf_counter()
+        W_prev = w_points(prob, np.array([0]))[0]
+        Y = ldmd.initialize_inelastic(W_prev[None] - 2 * prob.mu * E[:, None, None] * np.eye(prob.n_channel))
+        n = 0
+        while n < n_max and (time.perf_counter() - t0 < budget or n < 1):
+            Wm, We = w_points(prob, np.array([2 * n + 1, 2 * n + 2]))
+            ref, Qs, Qm, Qe = ldmd.correction_matrices(hs[n], W_prev, Wm, We)
+            for e in range(E.size):
+                Y[e] = ldmd.sector_step(Y[e], hs[n], ref - shift[e], Qs, Qm, Qe)
+            W_prev = We
+            n += 1
+        return n, time.perf_counter() - t0
+
+    n, dt = run(min(max_sectors, len(hs)), min_seconds / repeats)
+    times = [dt] + [run(n, float("inf"))[1] for _ in range(repeats - 1)]
+    rates = sorted(E.size * n / t for t in times)
     return {
-        "value": E.size * n / dt,
+        "value": rates[len(rates) // 2],
         "unit": "energy.R-steps/s",
         "cores": int(threads),
         "kind": "port",
-        "sample": f"NumPy oracle LDMD incl. W contraction, N={prob.n_channel}, {E.size} energies x first {n} sectors ({dt:.1f} s); "
-        f"host cpus={len(os.sched_getaffinity(0))}, BLAS threads={threads}",
+        "sample": f"NumPy oracle LDMD incl. W contraction, N={prob.n_channel}, {E.size} energies x first {n} sectors, "
+        f"median of {repeats} runs ({', '.join(f'{t:.1f}' for t in times)} s); host cpus={len(os.sched_getaffinity(0))}, "
+        f"BLAS threads={threads}",
     }
 
 
