@@ -542,7 +542,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // registers, and the next tile's C rows (into the idle accumulator set) and B
 // rows (into the idle LDS buffer) are loaded while the current tile's MFMAs
 // and stores run.
-template <int KB, int NBANDS>
+template <int KB, int NBANDS, bool ONE = false>
 __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, double* __restrict__ Out,
                                                long long ostride, int ldo, int ocol0, int Np, int k0,
                                                const double* __restrict__ AT, const int* __restrict__ src2,
@@ -672,6 +672,9 @@ __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, dou
   load_c(jt0, true, accA);
   store_b(0);
   __syncthreads();
+  if constexpr (ONE) {  // a single column tile: nothing to prefetch, one B buffer
+    compute_store(jt0, 0, accA);
+  } else {
   for (int J = jt0;; J += 2) {
     // even tile: accumulators A, LDS buffer 0; prefetch into B / buffer 1
     const bool more1 = J + 1 < jt1;
@@ -689,6 +692,7 @@ __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, dou
     if (!more2) break;
     store_b(0);
     __syncthreads();
+  }
   }
 }
 
@@ -711,6 +715,18 @@ __global__ __launch_bounds__(256 * NBANDS) __attribute__((amdgpu_waves_per_eu(2,
   if (jt0 >= jt1) return;
   gj_update_body<KB, NBANDS>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, blockIdx.y,
                              NBANDS * (blockIdx.x / ngroups), jt0, jt1, threadIdx.x, sB);
+}
+
+// One 64-column tile per workgroup (the look-ahead of the next pair's columns):
+// without the next tile's prefetch registers and B buffer the kernel fits four
+// waves per SIMD, so more of its latency-bound workgroups run at once.
+template <int KB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gj_update_one(
+    const double* __restrict__ X, double* __restrict__ Out, long long ostride, int ldo, int ocol0, int Np, int k0,
+    const double* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0) {
+  __shared__ double sB[1][KB][64];
+  gj_update_body<KB, 1, true>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, blockIdx.y, blockIdx.x, T0, T0 + 1,
+                              threadIdx.x, sB);
 }
 
 // One launch with two jobs on disjoint data: the first `npanel` workgroups run
@@ -850,6 +866,12 @@ int launch_update(const double* cur, double* out, long long ostride, int ldo, in
   constexpr int NBANDS = 1;
   const int units = ((Np + 63) / 64 + NBANDS - 1) / NBANDS;  // band groups of NBANDS bands
   const int span = T1 - T0;
+  if (span == 1) {
+    hipLaunchKernelGGL((k_gj_update_one<KB>), dim3(units, batch), dim3(256), 0, stream, cur, out, ostride, ldo, ocol0,
+                       Np, k0, AT, src2, brow, T0);
+    TICC_CHECK_LAUNCH();
+    return TICC_OK;
+  }
   // about walk / NBANDS workgroups: each walks span/ngroups tiles
   const int walk = t_walk_update > 0 ? t_walk_update : kWalkUpdate;
   int ngroups = (walk / NBANDS + units * batch - 1) / (units * batch);
