@@ -128,18 +128,25 @@ __device__ __forceinline__ double yend_elem(const double* __restrict__ X2e, cons
   return ye - y2e[i] * (x * y2e[j]);
 }
 
+// One workgroup per pair of mirror 32 x 32 tiles (ti <= tj): both tiles of Ye
+// are formed once into LDS and both tiles of Y written, so X2 is read once.
 __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, const int* __restrict__ pinv,
                                                const double* __restrict__ We, const double* __restrict__ Wm,
                                                double h, const double* __restrict__ y1, const double* __restrict__ y2,
                                                int N, int Np, int half, double* __restrict__ Y, int item0, int per,
                                                long long gstride) {
-  __shared__ double tile[32][33];
+  __shared__ double E1[32][33];
+  __shared__ double E2[32][33];
   const int e = blockIdx.y;
   We += block_off(item0, e, per, gstride);
   Wm += block_off(item0, e, per, gstride);
-  const int tn = Np / 32;
-  const int ti = blockIdx.x / tn;
-  const int tj = blockIdx.x % tn;
+  // pair index -> (ti, tj), 0 <= ti <= tj
+  const int pidx = blockIdx.x;
+  int tj = (int)((sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
+  while (tj * (tj + 1) / 2 > pidx) --tj;
+  while ((tj + 1) * (tj + 2) / 2 <= pidx) ++tj;
+  const int ti = pidx - tj * (tj + 1) / 2;
+  const bool mirror = ti != tj;
   const size_t n2 = (size_t)Np * Np;
   const double* X2e = X2 + e * n2;
   const int* pinve = pinv + (size_t)e * Np;
@@ -147,23 +154,35 @@ __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, co
   const double* y2e = y2 + (size_t)e * Np;
   const int tx = threadIdx.x & 31;
   const int ty = threadIdx.x >> 5;  // 0..7
-  // transposed tile: element (tj*32 + r, ti*32 + tx) stored at tile[tx][r]
   for (int r = ty; r < 32; r += 8) {
-    const int i = tj * 32 + r;
-    const int j = ti * 32 + tx;
-    tile[tx][r] = (i < N && j < N) ? yend_elem(X2e, pinve, We, Wm, h, y1e, y2e, N, Np, i, j) : 0.0;
+    const int i = ti * 32 + r, j = tj * 32 + tx;
+    E1[r][tx] = (i < N && j < N) ? yend_elem(X2e, pinve, We, Wm, h, y1e, y2e, N, Np, i, j) : 0.0;
+    if (mirror) {
+      const int i2 = tj * 32 + r, j2 = ti * 32 + tx;
+      E2[r][tx] = (i2 < N && j2 < N) ? yend_elem(X2e, pinve, We, Wm, h, y1e, y2e, N, Np, i2, j2) : 0.0;
+    }
   }
   __syncthreads();
+  double (*Et)[33] = mirror ? E2 : E1;  // the mirror tile (tj, ti)
   for (int r = ty; r < 32; r += 8) {
-    const int i = ti * 32 + r;
-    const int j = tj * 32 + tx;
+    const int i = ti * 32 + r, j = tj * 32 + tx;
     double v = 0.0;
     if (i < N && j < N) {
-      const double a = yend_elem(X2e, pinve, We, Wm, h, y1e, y2e, N, Np, i, j);
+      const double a = E1[r][tx], t = Et[tx][r];
       const bool flip = half > 0 && ((i < half) != (j < half));
-      v = 0.5 * (flip ? a - tile[r][tx] : a + tile[r][tx]);
+      v = 0.5 * (flip ? a - t : a + t);
     }
     Y[e * n2 + (size_t)i * Np + j] = v;
+    if (mirror) {
+      const int i2 = tj * 32 + r, j2 = ti * 32 + tx;
+      double v2 = 0.0;
+      if (i2 < N && j2 < N) {
+        const double a = E2[r][tx], t = E1[tx][r];
+        const bool flip = half > 0 && ((i2 < half) != (j2 < half));
+        v2 = 0.5 * (flip ? a - t : a + t);
+      }
+      Y[e * n2 + (size_t)i2 * Np + j2] = v2;
+    }
   }
 }
 
@@ -415,8 +434,8 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
         TICC_CHECK_LAUNCH();
         double* r2 = nullptr;
         TICC_RETURN_IF(gj_invert_raw(other, r1, ne, Np, &r2, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src), I(sc.piv), st));
-        hipLaunchKernelGGL(k_form3, dim3(tn * tn, ne), dim3(256), 0, st, r2, I(sc.pinv), We, Wm, h, y1, y2, N, Np, half,
-                           Yg, e0, per_block, gstride);
+        hipLaunchKernelGGL(k_form3, dim3(tn * (tn + 1) / 2, ne), dim3(256), 0, st, r2, I(sc.pinv), We, Wm, h, y1, y2, N,
+                           Np, half, Yg, e0, per_block, gstride);
         TICC_CHECK_LAUNCH();
       }
       TICC_CHECK_HIP(hipEventRecord(ev_done[2 * g + qb], st));
