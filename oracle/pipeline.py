@@ -8,9 +8,10 @@ endpoints are reused, runner.py:258-263) -> ``src/pyticc/match/finalize.py:32-98
 The setup arrays (channel list, weighted basis rows B_K, U, E_int, ladder
 labels) come from the caller; they are pinned by their own known-answer tests.
 
-Composed end-to-end S of a realistic multi-channel system: parity unpinned by
-any reference vector (the reference holds none); pinned here only by
-invariants -- S = S^T, unitarity, free-particle S = 1, O(h^4) step convergence.
+Composed end-to-end S of a realistic multi-channel system: the reference holds
+no S vector for one; pinned by fourth-order convergence to the limit of an
+independent RK4 integration of the coupled equations on the same W points, and
+by the invariants S = S^T, unitarity, free-particle S = 1.
 """
 
 from __future__ import annotations
