@@ -128,18 +128,47 @@ __device__ __forceinline__ double yend_elem(const double* __restrict__ X2e, cons
   return ye - y2e[i] * (x * y2e[j]);
 }
 
+// The next sector's A1 = (Y + Q_s) + diag y1 from element (i, j) of Y: the
+// arithmetic of k_form1, with the addition rounded on its own as when Y is
+// stored and read back.
+struct NextSector {
+  const double* Ws;  // W_start and W_mid of the next sector (block 0)
+  const double* Wm;
+  double h;
+  const double* y1;  // its y1 (batch x Np)
+  double* A;         // nullptr: write Y instead
+};
+
+__device__ __forceinline__ double next_a1(const NextSector& n, const double* __restrict__ Ws,
+                                          const double* __restrict__ Wm, const double* __restrict__ y1e, int N,
+                                          int i, int j, double v) {
+#pragma clang fp contract(off)
+  if (i >= N || j >= N) return (i == j) ? 1.0 : 0.0;
+  double t = Ws[(size_t)i * N + j];
+  if (i == j) t = t - Wm[(size_t)i * N + i];
+  const double qs = n.h * t / 3.0;
+  double a = v + qs;
+  if (i == j) a += y1e[i];
+  return a;
+}
+
 // One workgroup per pair of mirror 32 x 32 tiles (ti <= tj): both tiles of Ye
 // are formed once into LDS and both tiles of Y written, so X2 is read once.
+// With nx.A set, the next sector's A1 is written instead of Y (k_form1 fused).
 __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, const int* __restrict__ pinv,
                                                const double* __restrict__ We, const double* __restrict__ Wm,
                                                double h, const double* __restrict__ y1, const double* __restrict__ y2,
                                                int N, int Np, int half, double* __restrict__ Y, int item0, int per,
-                                               long long gstride) {
+                                               long long gstride, NextSector nx) {
   __shared__ double E1[32][33];
   __shared__ double E2[32][33];
   const int e = blockIdx.y;
   We += block_off(item0, e, per, gstride);
   Wm += block_off(item0, e, per, gstride);
+  const double* nWs = nx.Ws + (nx.A ? block_off(item0, e, per, gstride) : 0);
+  const double* nWm = nx.Wm + (nx.A ? block_off(item0, e, per, gstride) : 0);
+  const double* ny1 = nx.y1 + (size_t)e * Np;
+  double* Out = nx.A ? nx.A : Y;
   // pair index -> (ti, tj), 0 <= ti <= tj
   const int pidx = blockIdx.x;
   int tj = (int)((sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
@@ -172,7 +201,7 @@ __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, co
       const bool flip = half > 0 && ((i < half) != (j < half));
       v = 0.5 * (flip ? a - t : a + t);
     }
-    Y[e * n2 + (size_t)i * Np + j] = v;
+    Out[e * n2 + (size_t)i * Np + j] = nx.A ? next_a1(nx, nWs, nWm, ny1, N, i, j, v) : v;
     if (mirror) {
       const int i2 = tj * 32 + r, j2 = ti * 32 + tx;
       double v2 = 0.0;
@@ -181,7 +210,7 @@ __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, co
         const bool flip = half > 0 && ((i2 < half) != (j2 < half));
         v2 = 0.5 * (flip ? a - t : a + t);
       }
-      Y[e * n2 + (size_t)i2 * Np + j2] = v2;
+      Out[e * n2 + (size_t)i2 * Np + j2] = nx.A ? next_a1(nx, nWs, nWm, ny1, N, i2, j2, v2) : v2;
     }
   }
 }
@@ -347,8 +376,9 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
     sc.piv = take(sizeof(int) * GJ_PIV_LISTS * batch * GJ_NB);
     sc.perm = take(sizeof(int) * batch * Np);
     sc.pinv = take(sizeof(int) * batch * Np);
-    sc.y1 = yv ? take(sizeof(double) * batch * Np) : 0;
-    sc.y2 = yv ? take(sizeof(double) * batch * Np) : 0;
+    // y1, y2 of two sectors: the next sector's are formed before this one's last pass
+    sc.y1 = yv ? take(sizeof(double) * 2 * batch * Np) : 0;
+    sc.y2 = yv ? take(sizeof(double) * 2 * batch * Np) : 0;
     return sc;
   };
   const Scratch sm = scratch(n_block * qc, false);
@@ -381,6 +411,9 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
 
   const double two_mu = 2.0 * mu;
   const int gelem = grid1((long long)n2);
+  // per group: the buffer holding the current sector's A1 and its GJ partner
+  double* a_in[MAX_GROUPS];
+  double* a_alt[MAX_GROUPS];
   const int tn = Np / 32;
   const int n_chunk = (nS + qc - 1) / qc;
   const long long qstride = (long long)qc * n2;  // one block's Q_mid chunk
@@ -410,32 +443,48 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
       const Scratch& sc = sg[g];
       double* A0 = D(sc.A0);
       double* A1 = D(sc.A1);
-      double* y1 = D(sc.y1);
-      double* y2 = D(sc.y2);
       double* Yg = Yp + (size_t)e0 * n2;
+      auto ys = [&](size_t o, int s) { return D(o) + (size_t)(s & 1) * per * Np; };
+      auto refvals = [&](int s) {
+        hipLaunchKernelGGL(k_refvals, dim3((ne * Np + 255) / 256), dim3(256), 0, st, hs_host[s], Wm0 + s * wstride, N,
+                           Np, dE + e0, two_mu, ne, ys(sc.y1, s), ys(sc.y2, s), e0, per_block, gstride);
+        return hipGetLastError();
+      };
       TICC_CHECK_HIP(hipStreamWaitEvent(st, ev_qm[qb], 0));
       for (int q = 0; q < nc; ++q) {
         const int s = s0 + q;
         const double h = hs_host[s];
-        const double* Ws = Ws0 + s * wstride;
         const double* Wm = Wm0 + s * wstride;
         const double* We = We0 + s * wstride;
-        hipLaunchKernelGGL(k_refvals, dim3((ne * Np + 255) / 256), dim3(256), 0, st, h, Wm, N, Np, dE + e0, two_mu, ne,
-                           y1, y2, e0, per_block, gstride);
-        TICC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_form1, dim3(gelem, ne), dim3(256), 0, st, Yg, Ws, Wm, h, y1, N, Np, A0, e0, per_block,
-                           gstride);
-        TICC_CHECK_LAUNCH();
+        double* y1 = ys(sc.y1, s);
+        double* y2 = ys(sc.y2, s);
+        if (s == 0) {  // later sectors' A1 comes from the previous sector's k_form3
+          TICC_CHECK_HIP(refvals(s));
+          hipLaunchKernelGGL(k_form1, dim3(gelem, ne), dim3(256), 0, st, Yg, Ws0, Wm, h, y1, N, Np, A0, e0, per_block,
+                             gstride);
+          TICC_CHECK_LAUNCH();
+          a_in[g] = A0;
+          a_alt[g] = A1;
+        }
         double* r1 = nullptr;
-        TICC_RETURN_IF(gj_invert_raw(A0, A1, ne, Np, &r1, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src), I(sc.piv), st));
+        TICC_RETURN_IF(
+            gj_invert_raw(a_in[g], a_alt[g], ne, Np, &r1, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src), I(sc.piv), st));
         double* other = (r1 == A0) ? A1 : A0;
         hipLaunchKernelGGL(k_form2, dim3(gelem, ne), dim3(256), 0, st, r1, I(sc.pinv), Qbuf + q * n2, y1, y2, N, Np,
                            other, e0, per_block, qstride);
         TICC_CHECK_LAUNCH();
         double* r2 = nullptr;
         TICC_RETURN_IF(gj_invert_raw(other, r1, ne, Np, &r2, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src), I(sc.piv), st));
+        NextSector nx{nullptr, nullptr, 0.0, y1, nullptr};
+        if (s + 1 < nS) {  // form the next sector's A1 in the buffer X2 does not occupy
+          TICC_CHECK_HIP(refvals(s + 1));
+          nx = NextSector{Ws0 + (s + 1) * wstride, Wm0 + (s + 1) * wstride, hs_host[s + 1], ys(sc.y1, s + 1),
+                          (r2 == A0) ? A1 : A0};
+          a_in[g] = nx.A;
+          a_alt[g] = r2;
+        }
         hipLaunchKernelGGL(k_form3, dim3(tn * (tn + 1) / 2, ne), dim3(256), 0, st, r2, I(sc.pinv), We, Wm, h, y1, y2, N,
-                           Np, half, Yg, e0, per_block, gstride);
+                           Np, half, Yg, e0, per_block, gstride, nx);
         TICC_CHECK_LAUNCH();
       }
       TICC_CHECK_HIP(hipEventRecord(ev_done[2 * g + qb], st));
