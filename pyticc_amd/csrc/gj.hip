@@ -748,7 +748,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const int u = blockIdx.x - npanel;
   const int band_pairs = ((Np + 63) / 64 + 1) / 2;
   const int per_b = band_pairs * ngroups;
-  const int b = u / per_b, rem = u % per_b;
+  // Workgroups are dealt to the 8 XCDs round robin (npanel is a multiple of 8
+  // here, so XCD = u % 8): the update workgroups of one matrix go to one XCD,
+  // whose L2 then serves its staged B rows to all of its band pairs.
+  int b, rem;
+  if ((npanel & 7) == 0) {
+    const int k = u >> 3;
+    b = (u & 7) + 8 * (k / per_b);
+    rem = k % per_b;
+  } else {
+    b = u / per_b;
+    rem = u % per_b;
+  }
   int jt0, jt1;
   group_tiles(T0, T1, rem % ngroups, ngroups, jt0, jt1);
   if (jt0 >= jt1) return;
