@@ -702,6 +702,23 @@ __device__ __forceinline__ void group_tiles(int T0, int T1, int grp, int ngroups
   jt1 = T0 + ((grp + 1) * (T1 - T0)) / ngroups;
 }
 
+// (matrix b, workgroup w within it) of a launch with grid (per matrix, matrices).
+// Workgroups are dealt to the 8 XCDs round robin in linear order; with a
+// matrix count that is a multiple of 8 all workgroups of one matrix share an
+// XCD, whose L2 then serves the B rows they all stage.
+__device__ __forceinline__ void xcd_matrix_map(int& b, int& w) {
+  const int per = gridDim.x;
+  const int lin = blockIdx.x + per * blockIdx.y;
+  if ((gridDim.y & 7) == 0) {
+    const int k = lin >> 3;
+    b = (lin & 7) + 8 * (k / per);
+    w = k % per;
+  } else {
+    b = blockIdx.y;
+    w = blockIdx.x;
+  }
+}
+
 // NBANDS = 1: 256 threads, one band per workgroup; NBANDS = 2: 512 threads, two
 // bands sharing each B tile (half the B-row traffic).
 template <int KB, int NBANDS>
@@ -710,11 +727,13 @@ __global__ __launch_bounds__(256 * NBANDS) __attribute__((amdgpu_waves_per_eu(2,
     const double* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1,
     int ngroups) {
   __shared__ double sB[2][KB][64];
+  int b, w;
+  xcd_matrix_map(b, w);
   int jt0, jt1;
-  group_tiles(T0, T1, blockIdx.x % ngroups, ngroups, jt0, jt1);
+  group_tiles(T0, T1, w % ngroups, ngroups, jt0, jt1);
   if (jt0 >= jt1) return;
-  gj_update_body<KB, NBANDS>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, blockIdx.y,
-                             NBANDS * (blockIdx.x / ngroups), jt0, jt1, threadIdx.x, sB);
+  gj_update_body<KB, NBANDS>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, b, NBANDS * (w / ngroups), jt0,
+                             jt1, threadIdx.x, sB);
 }
 
 // One 64-column tile per workgroup (the look-ahead of the next pair's columns):
@@ -725,8 +744,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const double* __restrict__ X, double* __restrict__ Out, long long ostride, int ldo, int ocol0, int Np, int k0,
     const double* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0) {
   __shared__ double sB[1][KB][64];
-  gj_update_body<KB, 1, true>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, blockIdx.y, blockIdx.x, T0, T0 + 1,
-                              threadIdx.x, sB);
+  int b, w;
+  xcd_matrix_map(b, w);
+  gj_update_body<KB, 1, true>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, b, w, T0, T0 + 1, threadIdx.x, sB);
 }
 
 // One launch with two jobs on disjoint data: the first `npanel` workgroups run
