@@ -88,3 +88,23 @@ def test_nonfinite_result_is_reported(gpu_device):
 def test_complex_W_is_rejected(gpu_device):
     with pytest.raises(TypeError):
         propagate_logD(np.array([[[1.0j]]]), [0.1], 1.0, [0.1], np.ones((1, 1, 1)) * 1j, np.ones((1, 1, 1)), np.ones((1, 1, 1)))
+
+
+@pytest.mark.parametrize("groups", ["2", "3"])
+def test_energy_groups_give_the_single_group_sweep(gpu_device, monkeypatch, groups):
+    """Energies swept in groups on separate streams (TICC_SWEEP_GROUPS) give the
+    one-group sweep bit for bit, over 37 sectors (two Q_mid chunks) whose start
+    and end points differ between neighbouring sectors."""
+    rng = np.random.default_rng(29)
+    N, n_E, n_S = 70, 7, 37
+    Ws, Wm, We = random_W_batch(rng, n_S, N, scale=20.0)
+    hs = rng.uniform(0.01, 0.03, n_S)
+    E = rng.uniform(0.5, 2.0, n_E)
+    mu = 3.0
+    Y0 = oldmd.initialize_inelastic(Ws[0][None] - 2 * mu * E[:, None, None] * np.eye(N))
+    one = _device.to_host(propagate_logD(Y0, E, mu, hs, Ws, Wm, We))
+    monkeypatch.setenv("TICC_SWEEP_GROUPS", groups)
+    grouped = _device.to_host(propagate_logD(Y0, E, mu, hs, Ws, Wm, We))
+    np.testing.assert_array_equal(grouped, one)
+    ref = oldmd.propagate(Y0, E, mu, hs, Ws, Wm, We)
+    np.testing.assert_allclose(grouped, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
