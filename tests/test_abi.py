@@ -35,3 +35,18 @@ def test_invalid_arguments_map_to_value_error():
         _backend.call("ticc_invert_batched", 1, 0, None, None)
     with pytest.raises(NotImplementedError):
         _backend.call("ticc_invert_batched", 1, 5000, None, None)
+
+
+def test_lane_and_walk_knobs_validate_without_gpu():
+    import pytest
+
+    with pytest.raises(ValueError, match="lane"):
+        _backend.call("ticc_set_lane", -1)
+    with pytest.raises(ValueError, match="lane"):
+        _backend.call("ticc_set_lane", 4)
+    _backend.call("ticc_set_lane", 0)
+    with pytest.raises(ValueError, match="walk"):
+        _backend.call("ticc_set_sweep_walk", -1, 0)
+    with pytest.raises(ValueError, match="walk"):
+        _backend.call("ticc_set_sweep_walk", 0, 70000)
+    _backend.call("ticc_set_sweep_walk", 0, 0)  # back to the built-in targets
