@@ -50,3 +50,24 @@ def test_lane_and_walk_knobs_validate_without_gpu():
     with pytest.raises(ValueError, match="walk"):
         _backend.call("ticc_set_sweep_walk", 0, 70000)
     _backend.call("ticc_set_sweep_walk", 0, 0)  # back to the built-in targets
+
+
+def test_sweep_argument_checks_run_before_any_device_work():
+    import ctypes
+
+    import numpy as np
+    import pytest
+
+    with pytest.raises(ValueError, match="At least one total energy"):
+        _backend.call("ticc_ldmd_propagate", 0, 4, None, 1.0, 0, None, None, None, None, 16, None, None)
+    E = np.array([0.1])
+    hs = np.array([0.0])
+    W = np.zeros((1, 4, 4))
+    Y = np.zeros((1, 4, 4))
+    w, y = W.ctypes.data, Y.ctypes.data  # host buffers: the checks reject the call before any use
+    with pytest.raises(ValueError, match="radial_half_step must be positive"):
+        _backend.call("ticc_ldmd_propagate", 1, 4, E.ctypes.data_as(_backend._dp), 1.0, 1,
+                      hs.ctypes.data_as(_backend._dp), w, w, w, 16, y, None)
+    with pytest.raises(NotImplementedError):
+        _backend.call("ticc_ldmd_propagate", 1, 1537, E.ctypes.data_as(_backend._dp), 1.0, 0, None, None, None,
+                      None, 16, ctypes.c_void_p(y), None)

@@ -58,7 +58,8 @@ def test_sweep_matches_sequential_sector_updates(gpu_device):
     np.testing.assert_allclose(res, oldmd.propagate(Y0, E, mu, hs, Ws, Wm, We), rtol=1e-12, atol=1e-12)
 
 
-@pytest.mark.parametrize("N,n_E,n_S", [(3, 2, 5), (33, 3, 7), (66, 4, 40), (130, 2, 6), (1024, 2, 2), (1200, 2, 2)])
+@pytest.mark.parametrize("N,n_E,n_S", [(3, 2, 5), (33, 3, 7), (66, 4, 40), (130, 2, 6), (1024, 2, 2), (1200, 2, 2),
+                                        (1536, 2, 2)])
 def test_sweep_matches_oracle_random_coupled(gpu_device, N, n_E, n_S):
     rng = np.random.default_rng(N * 7 + n_S)
     Ws, Wm, We = random_W_batch(rng, n_S, N, scale=20.0)
@@ -108,3 +109,13 @@ def test_energy_groups_give_the_single_group_sweep(gpu_device, monkeypatch, grou
     np.testing.assert_array_equal(grouped, one)
     ref = oldmd.propagate(Y0, E, mu, hs, Ws, Wm, We)
     np.testing.assert_allclose(grouped, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+
+
+def test_zero_sectors_leave_Y_unchanged(gpu_device):
+    rng = np.random.default_rng(5)
+    N = 9
+    Y0 = rng.standard_normal((3, N, N))
+    Y0 = Y0 + np.swapaxes(Y0, 1, 2)
+    empty = np.zeros((0, N, N))
+    got = _device.to_host(propagate_logD(Y0, [0.1, 0.2, 0.3], 2.0, [], empty, empty, empty))
+    np.testing.assert_array_equal(got, Y0)
