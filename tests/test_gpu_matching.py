@@ -99,3 +99,22 @@ def test_matching_matches_oracle_random_batch(gpu_device, N, E, tol):
     for g, r in zip(got, ref):
         assert g.shape == r.shape
         np.testing.assert_allclose(g, r, rtol=0, atol=tol)
+
+
+def test_ragged_batch_with_every_channel_closed(gpu_device):
+    """An energy below every threshold gives an empty S (0, 0) beside a (2, 2) one,
+    as the oracle's open/closed split (smatrix.py:56-57) does."""
+    rng = np.random.default_rng(1)
+    N = 5
+    A = rng.standard_normal((2, N, N))
+    Y = 0.5 * (A + np.swapaxes(A, 1, 2)) + np.diag(rng.uniform(1.0, 5.0, N))
+    E_int = np.array([0.3, 0.5, 0.7, 0.9, 1.1])
+    L = np.array([0.0, 1.0, 2.0, 3.0, 4.0])
+    E = np.array([0.1, 0.6])
+    S_re, S_im, n_open = get_Smat_device(Y, np.eye(N), 12.0, E, 3.0, E_int, L, gpu_device)
+    got = unpack_smat(S_re, S_im, n_open)
+    ref = omatch.get_smat(Y, 12.0, E, 3.0, E_int, L)
+    assert [g.shape for g in got] == [(0, 0), (2, 2)]
+    for g, r in zip(got, ref):
+        assert g.shape == r.shape
+        np.testing.assert_allclose(g, r, rtol=0, atol=1e-12)
