@@ -341,7 +341,7 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   const int qc = n_block == 1 ? QCHUNK : (QCHUNK / n_block > 0 ? QCHUNK / n_block : 1);
   // Energy groups may run on their own streams (TICC_SWEEP_GROUPS): with two,
   // one group's serial pieces (its first panel chain, look-ahead updates, form
-  // kernels) run beside the other group's fused launches, about 4% faster at
+  // kernels) run beside the other group's fused launches, 3-5% faster at
   // N = 800 and 400 with 64 energies (three or four groups are slower).  One
   // group is the default, so that each fused launch runs alone and its timing
   // measures the kernel itself.
