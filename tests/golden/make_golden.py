@@ -1,6 +1,6 @@
 """Generate the committed golden fixtures (run on the CPU; minutes for the N=800 case).
 
-    python tests/golden/make_golden.py [bessel] [cg] [config1] [config2]
+    python tests/golden/make_golden.py [bessel] [cg] [atom_rotor_N66] [rotor_rotor_N800]
 
 Fixtures (NumPy .npz, no pickled objects):
   bessel_mpmath.npz   Riccati-Bessel j, n, j', n' (j = sqrt(pi x/2) J_{l+1/2}(x)) and the
@@ -9,9 +9,9 @@ Fixtures (NumPy .npz, no pickled objects):
                       high-precision check of the scipy-based oracle and of the device code.
   cg_sympy.npz        Clebsch-Gordan coefficients from sympy.physics.wigner (exact
                       rationals under a square root, rounded once to FP64).
-  smat_config1.npz    oracle S-matrices of the atom + rigid-rotor system (N = 66, J = 10,
+  smat_atom_rotor_N66.npz    oracle S-matrices of the atom + rigid-rotor system (N = 66, J = 10,
                       p = +1, 1350 sectors 3-30 bohr) at 5 of its 20 energies.
-  smat_config2.npz    oracle S-matrices of the two-rotor system (N = 800, J = 4, p = -1,
+  smat_rotor_rotor_N800.npz    oracle S-matrices of the two-rotor system (N = 800, J = 4, p = -1,
                       2000 sectors 4-40 bohr, 15 x 15 x 12 grid) at 3 of its 64 energies:
                       the first 16 columns of S, its diagonal and the oracle's unitarity defect.
 
@@ -88,7 +88,7 @@ def _oracle_smat(workload, energy_index):
     return E, S, [omatch.unitarity_defect(s) if s.size else 0.0 for s in S]
 
 
-def make_config1() -> None:
+def make_atom_rotor_N66() -> None:
     from pyticc_amd import synthetic
 
     w = synthetic.atom_rotor()
@@ -96,10 +96,10 @@ def make_config1() -> None:
     out = {"energy_index": np.array(CONFIG1_ENERGIES), "energies": E, "defect": np.array(defect)}
     for i, s in enumerate(S):
         out[f"S{i}"] = s
-    np.savez_compressed(HERE / "smat_config1.npz", **out)
+    np.savez_compressed(HERE / "smat_atom_rotor_N66.npz", **out)
 
 
-def make_config2() -> None:
+def make_rotor_rotor_N800() -> None:
     from pyticc_amd import synthetic
 
     w = synthetic.rotor_rotor()
@@ -109,12 +109,12 @@ def make_config2() -> None:
     for i, s in enumerate(S):
         out[f"S{i}_cols"] = s[:, :CONFIG2_COLUMNS]
         out[f"S{i}_diag"] = np.diag(s).copy()
-    np.savez_compressed(HERE / "smat_config2.npz", **out)
+    np.savez_compressed(HERE / "smat_rotor_rotor_N800.npz", **out)
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["bessel", "cg", "config1", "config2"]
+    which = sys.argv[1:] or ["bessel", "cg", "atom_rotor_N66", "rotor_rotor_N800"]
     for name in which:
         t = time.perf_counter()
-        {"bessel": make_bessel, "cg": make_cg, "config1": make_config1, "config2": make_config2}[name]()
+        {"bessel": make_bessel, "cg": make_cg, "atom_rotor_N66": make_atom_rotor_N66, "rotor_rotor_N800": make_rotor_rotor_N800}[name]()
         print(f"{name}: {time.perf_counter() - t:.1f} s", flush=True)

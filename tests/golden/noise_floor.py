@@ -2,7 +2,7 @@
 
 Runs the oracle with explicit inverses in the LDMD sector step instead of linear
 solves -- an equally valid FP64 restatement of the same algorithm -- at two of the
-energies of tests/golden/smat_config2.npz and prints the largest |dS| against the
+energies of tests/golden/smat_rotor_rotor_N800.npz and prints the largest |dS| against the
 fixture.  Measured: 4.2e-11 (energy index 31) and 4.2e-10 (energy index 63).
 
     python tests/golden/noise_floor.py
@@ -31,7 +31,7 @@ def sector_step_inv(Y, h, ref, Qs, Qm, Qe):
     return 0.5 * (Yend + Yend.T)
 
 ldmd.sector_step = sector_step_inv
-d = np.load(Path(__file__).resolve().parent / 'smat_config2.npz')
+d = np.load(Path(__file__).resolve().parent / 'smat_rotor_rotor_N800.npz')
 w = synthetic.rotor_rotor()
 _, _, prob = oracle_problem(w)
 idx = [int(d['energy_index'][1]), int(d['energy_index'][2])]

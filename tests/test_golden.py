@@ -40,8 +40,8 @@ def test_clebsch_gordan_matches_sympy():
     np.testing.assert_allclose(got, d["value"], rtol=0, atol=1e-14)
 
 
-def test_oracle_reproduces_config1_fixture():
-    d = np.load(GOLDEN / "smat_config1.npz")
+def test_oracle_reproduces_atom_rotor_N66_fixture():
+    d = np.load(GOLDEN / "smat_atom_rotor_N66.npz")
     w = synthetic.atom_rotor()
     _, _, prob = oracle_problem(w)
     idx = d["energy_index"]

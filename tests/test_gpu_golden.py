@@ -35,8 +35,8 @@ def test_device_bessel_matches_mpmath(gpu_device):
     np.testing.assert_allclose(k_logd_batch(d["nu"], d["xk"], gpu_device.index), d["k_logd"], rtol=1e-13)
 
 
-def test_solve_config1_matches_fixture(gpu_device):
-    d = np.load(GOLDEN / "smat_config1.npz")
+def test_solve_atom_rotor_N66_matches_fixture(gpu_device):
+    d = np.load(GOLDEN / "smat_atom_rotor_N66.npz")
     w = synthetic.atom_rotor()
     grid = ticc.prepare_potential(w.system, w.boundaries, w.half_steps, **w.quadrature)
     res = ticc.solve(w.system, w.energies, grid, ticc.Propagation(device="gpu"))
@@ -54,8 +54,8 @@ def test_solve_config1_matches_fixture(gpu_device):
     np.testing.assert_allclose(vals, res.Smat[e].T.ravel(), rtol=1e-15, atol=1e-16)
 
 
-def test_solve_config2_full_size_matches_fixture(gpu_device):
-    path = GOLDEN / "smat_config2.npz"
+def test_solve_rotor_rotor_N800_full_size_matches_fixture(gpu_device):
+    path = GOLDEN / "smat_rotor_rotor_N800.npz"
     d = np.load(path)
     w = synthetic.rotor_rotor()
     idx = d["energy_index"]
@@ -72,5 +72,5 @@ def test_solve_config2_full_size_matches_fixture(gpu_device):
         worst = max(worst, float(np.max(np.abs(np.diag(S) - d[f"S{i}_diag"]))))
         # the device S is as unitary as the oracle's on the same inputs
         assert omatch.unitarity_defect(S) <= 10.0 * float(d["defect"][i]) + 1e-12
-    print(f"config-2 full size: max |dS| = {worst:.3e}, oracle defects {d['defect']}")
+    print(f"N = 800 two-rotor full size: max |dS| = {worst:.3e}, oracle defects {d['defect']}")
     assert worst <= TOL_S_FULL, worst
