@@ -89,6 +89,29 @@ def sector_step(Y, h, ref, Qs, Qm, Qe):
     return 0.5 * (Yend + Yend.T)
 
 
+def sector_step_inverse(Y, h, ref, Qs, Qm, Qe):
+    """The same sector with A^-1 formed explicitly (LAPACK getri) and then scaled,
+    instead of solved against diag(y2) -- an equally valid FP64 ordering of
+    logd.py:168-194.  The distance between the S-matrices of the two orderings is
+    the FP64 noise floor of an end-to-end comparison."""
+    y1, y2 = reference_values(h, ref)
+    n = Y.shape[-1]
+    idx = np.arange(n)
+    A1 = Y + Qs
+    A1[idx, idx] += y1
+    X1 = np.linalg.inv(A1) * y2[None, :]
+    Ymid = Qm.copy()
+    Ymid[idx, idx] += y1
+    Ymid -= y2[:, None] * X1
+    A2 = Ymid + Qm
+    A2[idx, idx] += y1
+    X2 = np.linalg.inv(A2) * y2[None, :]
+    Yend = Qe.copy()
+    Yend[idx, idx] += y1
+    Yend -= y2[:, None] * X2
+    return 0.5 * (Yend + Yend.T)
+
+
 def propagate(Y0, energies, mu, half_steps, Wb_start, Wb_mid, Wb_end):
     """Energy batch x sector scan (logd.py:288-330, 340-398).
 

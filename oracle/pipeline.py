@@ -50,29 +50,41 @@ def w_points(p: Problem, rows: np.ndarray) -> np.ndarray:
     return contraction.w_base(R, p.mu, p.E_int, p.U, Vm)
 
 
-def propagate(p: Problem, energies, sectors_limit: int | None = None, mode: str = "inelastic") -> np.ndarray:
-    """Y_BF after all (or the first ``sectors_limit``) sectors, shape (n_E, N, N)."""
+def propagate(p: Problem, energies, sectors_limit: int | None = None, mode: str = "inelastic",
+              steps=(ldmd.sector_step,)) -> list:
+    """Y_BF after all (or the first ``sectors_limit``) sectors, shape (n_E, N, N).
+
+    ``steps``: the sector-step orderings to run side by side on the same W points
+    (default: the solve-based step alone, returned as an array); with several,
+    a list of one Y per ordering."""
     E = np.asarray(energies, dtype=np.float64)
     n_s = len(p.sectors) if sectors_limit is None else min(sectors_limit, len(p.sectors))
     hs = grid.half_steps_of(p.sectors)
     W_prev = w_points(p, np.array([0]))[0]
     eye = np.eye(p.n_channel)
     W0 = W_prev[None] - 2.0 * p.mu * E[:, None, None] * eye
-    Y = ldmd.initialize_inelastic(W0) if mode == "inelastic" else ldmd.initialize_capture(W0)
+    Y0 = ldmd.initialize_inelastic(W0) if mode == "inelastic" else ldmd.initialize_capture(W0)
+    Ys = [Y0.copy() for _ in steps]
     shift = 2.0 * p.mu * E
     for s in range(n_s):
         Wm, We = w_points(p, np.array([2 * s + 1, 2 * s + 2]))
         ref, Qs, Qm, Qe = ldmd.correction_matrices(hs[s], W_prev, Wm, We)
-        for e in range(E.size):
-            Y[e] = ldmd.sector_step(Y[e], hs[s], ref - shift[e], Qs, Qm, Qe)
+        for Y, step in zip(Ys, steps):
+            for e in range(E.size):
+                Y[e] = step(Y[e], hs[s], ref - shift[e], Qs, Qm, Qe)
         W_prev = We
-    return Y
+    return Ys[0] if len(steps) == 1 else Ys
+
+
+def match(p: Problem, Y, energies):
+    """(B, L, S tuple) of propagated BF log derivatives (finalize.py:32-98)."""
+    B, L = matching.bmat_bf_to_sf(p.U, p.ladder, p.j_couple, p.Jtot)
+    Ysf = matching.transform_bf_to_sf(Y, B)
+    return B, L, matching.get_smat(Ysf, p.sectors[-1][1], energies, p.mu, p.E_int, L)
 
 
 def solve(p: Problem, energies, mode: str = "inelastic"):
     """(Y_BF, B, L, S tuple) of one exact-CC block (finalize.py:32-98)."""
     Y = propagate(p, energies, mode=mode)
-    B, L = matching.bmat_bf_to_sf(p.U, p.ladder, p.j_couple, p.Jtot)
-    Ysf = matching.transform_bf_to_sf(Y, B)
-    S = matching.get_smat(Ysf, p.sectors[-1][1], energies, p.mu, p.E_int, L)
+    B, L, S = match(p, Y, energies)
     return Y, B, L, S

@@ -31,6 +31,7 @@ from pyticc_amd.constants import (
     KHZ2AU,
     MHZ2AU,
 )
+from pyticc_amd.crosssection import PartialCrossSections, partial_cross_sections
 from pyticc_amd.energy import get_Etot
 from pyticc_amd.pes import PESWrapper
 from pyticc_amd.pes.native import load_native_pes
@@ -45,8 +46,8 @@ from pyticc_amd.system import Approx, ChannelSpec, MolInnerState, ScatteringType
 __all__ = [
     "AMU2AU", "ANG2AU", "AU2AMU", "AU2ANG", "AU2CM", "AU2EV", "AU2GHZ", "AU2HZ", "AU2K", "AU2KHZ", "AU2MHZ",
     "Approx", "AtomSpec", "CM2AU", "Channel", "ChannelBasis", "ChannelSpec", "CoupledStatesResult", "DiatomBasis",
-    "DiatomSpec", "EV2AU", "GHZ2AU", "HZ2AU", "K2AU", "KBlock", "KBlockResult", "KHZ2AU", "MHZ2AU", "MolInnerState", "PESWrapper", "PotentialGrid", "Propagation",
+    "DiatomSpec", "EV2AU", "GHZ2AU", "HZ2AU", "K2AU", "KBlock", "KBlockResult", "KHZ2AU", "MHZ2AU", "MolInnerState", "PartialCrossSections", "PESWrapper", "PotentialGrid", "Propagation",
     "RovibBasis", "ScatteringResult", "ScatteringType", "ScattHamiltonian", "ScattSystem", "Timing",
-    "build_ChannelBasis", "build_ScattSystem", "build_cs_blocks", "build_nncc_blocks", "element_mass_au", "get_Etot", "load_native_pes", "prepare_potential", "reduced_mass",
+    "build_ChannelBasis", "build_ScattSystem", "build_cs_blocks", "build_nncc_blocks", "element_mass_au", "get_Etot", "load_native_pes", "partial_cross_sections", "prepare_potential", "reduced_mass",
     "prepare_Diatom", "report", "rigid_rotor", "run", "solve",
 ]

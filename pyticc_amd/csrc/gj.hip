@@ -24,6 +24,7 @@
 // Row swaps compose into perm (original row at each position); the inverse of
 // the matrix is Out with its columns gathered through pinv = perm^-1, which the
 // consumers fuse into their own reads.
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
@@ -805,7 +806,7 @@ __global__ void k_unpermute(const double* __restrict__ in, double* __restrict__ 
 // algorithmic bytes and flop.
 struct UpdateProfile {
   std::mutex mu;
-  bool on = false;
+  std::atomic<bool> on{false};
   int every = 1;
   long long calls = 0;
   std::vector<hipEvent_t> events;  // pairs: [2i] start, [2i+1] stop
@@ -991,8 +992,10 @@ int single_step(double* cur, double* nxt, int batch, int Np, int k0, const Carve
 
 // Fused launch: the panel chain P beside column tiles [T0, T1) of the rank-2NB
 // update of the pair at k0.
+// `slot` >= 0: the profile slot reserved for this launch (its events, bytes and
+// flop entries), -1: not timed.
 int launch_fused(const PairJob& P, const double* X, double* Out, int batch, int Np, int k0, const double* AT,
-                 const int* src2, const int* brow, int T0, int T1, bool sample, hipStream_t stream) {
+                 const int* src2, const int* brow, int T0, int T1, long long slot, hipStream_t stream) {
   const int band_pairs = ((Np + 63) / 64 + 1) / 2;
   // about two workgroups per CU: longer tile walks amortise each band's
   // prologue (A fragments, first C and B tiles) better than more, shorter ones
@@ -1001,14 +1004,12 @@ int launch_fused(const PairJob& P, const double* X, double* Out, int batch, int 
   if (ngroups > T1 - T0) ngroups = T1 - T0;
   if (ngroups < 1) ngroups = 1;
   const int grid = batch + band_pairs * ngroups * batch;
-  size_t slot = 0;
-  if (sample) {
+  if (slot >= 0) {
     // algorithmic work: the update reads X and writes the partner buffer on its
     // columns and reads the 2NB coefficient columns; the panel chain reads its
     // 2NB (or NB) look-ahead columns and writes 2NB coefficient columns
     const double cols = (double)((T1 * 64 < Np ? T1 * 64 : Np) - T0 * 64);
     const double kp = P.single ? NB : 2 * NB;
-    slot = g_prof.used++;
     g_prof.bytes[slot] = 8.0 * batch * (2.0 * Np * cols + 2.0 * NB * Np + 2.0 * kp * Np);
     g_prof.flop[slot] = 2.0 * batch * Np * cols * 2 * NB;
     TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
@@ -1023,7 +1024,7 @@ int launch_fused(const PairJob& P, const double* X, double* Out, int batch, int 
     hipLaunchKernelGGL((k_gj_fused<3>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
                        T0, T1, ngroups);
   TICC_CHECK_LAUNCH();
-  if (sample) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
+  if (slot >= 0) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
   return TICC_OK;
 }
 }  // namespace
@@ -1036,14 +1037,21 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
   const long long n2 = (long long)Np * Np;
   const int tiles = (Np + 63) / 64;
   double* buf[2] = {bufA, bufB};
-  bool sample = false;
-  if (g_prof.on) {
-    std::lock_guard<std::mutex> lock(g_prof.mu);
-    sample = g_prof.on && (g_prof.calls++ % g_prof.every == 0) &&
-             (g_prof.used + (size_t)(Np / NB) <= g_prof.bytes.size());
-  }
   const int steps = Np / NB;
   const int pairs = steps / 2;
+  // Profile slots of this inversion's fused launches (one per pair but the last,
+  // plus the last pair's when a trailing single step follows), reserved under
+  // the lock so that inversions on concurrent lanes never share a slot.
+  const int n_fused = pairs > 0 ? pairs - 1 + (steps & 1) : 0;
+  long long slot = -1;
+  if (g_prof.on && n_fused > 0) {
+    std::lock_guard<std::mutex> lock(g_prof.mu);
+    if (g_prof.on && (g_prof.calls++ % g_prof.every == 0) && g_prof.used + (size_t)n_fused <= g_prof.bytes.size()) {
+      slot = (long long)g_prof.used;
+      g_prof.used += (size_t)n_fused;
+    }
+  }
+  auto next_slot = [&]() { return slot < 0 ? -1LL : slot++; };
   const long long ln = 2LL * NB * Np;  // look-ahead buffer: Np x 2NB per matrix
   if (pairs > 0) TICC_RETURN_IF(launch_pair_chain(pair_job(buf[0], n2, Np, 0, Np, 0, c, 0, perm, pinv), batch, stream));
   for (int q = 0; q < pairs; ++q) {
@@ -1065,7 +1073,7 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
       PairJob P = pair_job(c.L, ln, 2 * NB, 0, Np, ks, c, par ^ 1, perm, pinv);
       P.single = 1;
       TICC_RETURN_IF(
-          launch_fused(P, cur, nxt, batch, Np, k0, c.AT[par], c.src2[par], c.brow[par], 0, tiles, sample, stream));
+          launch_fused(P, cur, nxt, batch, Np, k0, c.AT[par], c.src2[par], c.brow[par], 0, tiles, next_slot(), stream));
       TICC_RETURN_IF(launch_update<NB>(nxt, cur, n2, Np, 0, batch, Np, ks, c.AT[par ^ 1], c.src2[par ^ 1],
                                        c.brow[par ^ 1], 0, tiles, stream));
       *result = cur;
@@ -1077,7 +1085,7 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
     TICC_RETURN_IF(launch_update<2 * NB>(cur, c.L, ln, 2 * NB, kn, batch, Np, k0, c.AT[par], c.src2[par],
                                          c.brow[par], kn / 64, kn / 64 + 1, stream));
     TICC_RETURN_IF(launch_fused(pair_job(c.L, ln, 2 * NB, 0, Np, kn, c, par ^ 1, perm, pinv), cur, nxt, batch, Np, k0,
-                                c.AT[par], c.src2[par], c.brow[par], 0, tiles, sample, stream));
+                                c.AT[par], c.src2[par], c.brow[par], 0, tiles, next_slot(), stream));
   }
   double* cur = buf[pairs & 1];
   if (steps & 1) {  // trailing single step: rank-NB update

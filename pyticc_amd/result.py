@@ -27,6 +27,7 @@ class ScatteringResult:
     L: np.ndarray
     Smat: tuple
     timing: Timing | None = None
+    reduced_mass: float | None = None  # this build's addition: cross sections need k_a
 
     @property
     def molecule_exchange(self) -> int:
@@ -35,6 +36,12 @@ class ScatteringResult:
     @property
     def open_closed(self):
         return self.basis.open_closed(self.Etot)
+
+    def partial_cross_sections(self):
+        """Per energy, sigma^{Jp}(a -> b) between the open monomer levels (bohr^2)."""
+        from pyticc_amd.crosssection import partial_cross_sections
+
+        return partial_cross_sections(self)
 
     @property
     def Y_asymptotic(self) -> np.ndarray:

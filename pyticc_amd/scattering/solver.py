@@ -52,7 +52,8 @@ def solve(problem, Etot, potential_grid, propagation) -> ScatteringResult:
     logger.info(f"Solving Jtot={basis.Jtot}, system_parity={basis.system_parity:+d}, approx=exact, channels={basis.n_channel}, energies={energies.size}")
     B, L = get_Bmat_BF_to_SF(basis)
     Y, Smat = solve_device(hamiltonian, energies, sectors, propagation, B, L)
-    result = ScatteringResult(basis=basis, Etot=energies, Y_propagated=Y, asymptotic_transform=B, L=L, Smat=Smat)
+    result = ScatteringResult(basis=basis, Etot=energies, Y_propagated=Y, asymptotic_transform=B, L=L, Smat=Smat,
+                              reduced_mass=hamiltonian.reduced_mass)
     timing = Timing(wall_seconds=perf_counter() - wall0, cpu_seconds=process_time() - cpu0)
     logger.info(f"Solver complete: {timing}")
     return replace(result, timing=timing)

@@ -12,8 +12,14 @@ Fixtures (NumPy .npz, no pickled objects):
   smat_atom_rotor_N66.npz    oracle S-matrices of the atom + rigid-rotor system (N = 66, J = 10,
                       p = +1, 1350 sectors 3-30 bohr) at 5 of its 20 energies.
   smat_rotor_rotor_N800.npz    oracle S-matrices of the two-rotor system (N = 800, J = 4, p = -1,
-                      2000 sectors 4-40 bohr, 15 x 15 x 12 grid) at 3 of its 64 energies:
-                      the first 16 columns of S, its diagonal and the oracle's unitarity defect.
+                      2000 sectors 4-40 bohr, 15 x 15 x 12 grid) at 6 of its 64 energies:
+                      every element of S at energies 4, 15 and 31 (17, 190 and 473 open
+                      channels; the upper triangle, S being symmetric), the diagonal and 64
+                      seeded random columns at energies 47 and 63 (729 and 800 open), none
+                      open at energy 0; per energy the oracle's unitarity defect and the FP64
+                      noise floor -- max |dS| between the solve-based oracle and the same
+                      oracle with explicit inverses in the sector step
+                      (oracle.ldmd.sector_step_inverse), two equally valid FP64 orderings.
 
 Every fixture is written by the oracle (tests/../oracle) or by mpmath/sympy; nothing
 here reads the reference sources.
@@ -31,8 +37,9 @@ HERE = Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE.parents[1]))
 
 CONFIG1_ENERGIES = (0, 4, 9, 14, 19)
-CONFIG2_ENERGIES = (0, 31, 63)
-CONFIG2_COLUMNS = 16
+CONFIG2_ENERGIES = (0, 4, 15, 31, 47, 63)
+CONFIG2_FULL_MAX_OPEN = 480  # energies with at most this many open channels store all of S
+CONFIG2_SAMPLE_COLUMNS = 64  # else: the diagonal and this many seeded random columns
 
 
 def make_bessel() -> None:
@@ -100,15 +107,33 @@ def make_atom_rotor_N66() -> None:
 
 
 def make_rotor_rotor_N800() -> None:
+    from oracle import ldmd
+    from oracle import matching as omatch
+    from oracle import pipeline as opipe
     from pyticc_amd import synthetic
+    from tests.helpers import oracle_problem
 
     w = synthetic.rotor_rotor()
-    E, S, defect = _oracle_smat(w, CONFIG2_ENERGIES)
-    out = {"energy_index": np.array(CONFIG2_ENERGIES), "energies": E, "defect": np.array(defect),
+    _, _, prob = oracle_problem(w)
+    E = w.energies[list(CONFIG2_ENERGIES)]
+    Y_solve, Y_inv = opipe.propagate(prob, E, steps=(ldmd.sector_step, ldmd.sector_step_inverse))
+    _, _, S = opipe.match(prob, Y_solve, E)
+    _, _, S_inv = opipe.match(prob, Y_inv, E)
+    rng = np.random.default_rng(800)
+    out = {"energy_index": np.array(CONFIG2_ENERGIES), "energies": E,
+           "defect": np.array([omatch.unitarity_defect(s) if s.size else 0.0 for s in S]),
+           "floor": np.array([float(np.max(np.abs(a - b))) if a.size else 0.0 for a, b in zip(S, S_inv)]),
            "n_open": np.array([s.shape[0] for s in S])}
     for i, s in enumerate(S):
-        out[f"S{i}_cols"] = s[:, :CONFIG2_COLUMNS]
-        out[f"S{i}_diag"] = np.diag(s).copy()
+        n = s.shape[0]
+        if n <= CONFIG2_FULL_MAX_OPEN:
+            iu = np.triu_indices(n)
+            out[f"S{i}_upper"] = s[iu]
+        else:
+            cols = np.sort(rng.choice(n, CONFIG2_SAMPLE_COLUMNS, replace=False))
+            out[f"S{i}_colidx"] = cols
+            out[f"S{i}_cols"] = s[:, cols]
+            out[f"S{i}_diag"] = np.diag(s).copy()
     np.savez_compressed(HERE / "smat_rotor_rotor_N800.npz", **out)
 
 

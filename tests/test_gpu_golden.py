@@ -2,10 +2,12 @@
 S-matrices of the atom + rigid-rotor system (N = 66, all 1350 sectors) and of the
 two-rotor benchmark system at its full size (N = 800, 2000 sectors).
 
-Tolerance at full size: two equally valid FP64 restatements of the same algorithm
-differ there by up to 4.2e-10 in S (the oracle with explicit inverses instead of
-solves, tests/golden/noise_floor.py, measured at the fixture's energies), so the
-full-size comparison uses 1e-9; the reduced systems use 1e-10."""
+Tolerance at full size: per energy, the 1e-10 bar or twice the FP64 noise floor
+recorded with the fixture, whichever is larger.  The floor is the distance between
+two equally valid FP64 orderings of the same algorithm -- the oracle solving each
+sector against diag(y2) and the same oracle forming the inverse explicitly
+(oracle.ldmd.sector_step_inverse) -- after 2000 sectors at N = 800; the reduced
+systems use 1e-10."""
 
 from pathlib import Path
 
@@ -16,12 +18,12 @@ import pyticc_amd as ticc
 from oracle import matching as omatch
 from pyticc_amd import synthetic
 from pyticc_amd.match.bessel import k_logd_batch, riccati_bessel_batch
+from tests.helpers import n800_tolerance, rotor_rotor_N800_reference
 
 pytestmark = pytest.mark.gpu
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 TOL_S = 1.0e-10
-TOL_S_FULL = 1.0e-9  # FP64 noise floor of the N = 800, 2000-sector system: 4.2e-10
 
 
 def test_device_bessel_matches_mpmath(gpu_device):
@@ -61,16 +63,18 @@ def test_solve_rotor_rotor_N800_full_size_matches_fixture(gpu_device):
     idx = d["energy_index"]
     grid = ticc.prepare_potential(w.system, w.boundaries, w.half_steps, **w.quadrature)
     res = ticc.solve(w.system, w.energies[idx], grid, ticc.Propagation(device="gpu", memory_mb=4096.0))
-    worst = 0.0
+    report = []
     for i in range(idx.size):
         S = res.Smat[i]
         assert S.shape[0] == int(d["n_open"][i])
         if S.size == 0:  # no open channel at this energy
             continue
-        cols = d[f"S{i}_cols"]
-        worst = max(worst, float(np.max(np.abs(S[:, : cols.shape[1]] - cols))))
-        worst = max(worst, float(np.max(np.abs(np.diag(S) - d[f"S{i}_diag"]))))
+        r, c, ref = rotor_rotor_N800_reference(d, i)
+        err = float(np.max(np.abs(S[r, c] - ref)))
+        tol = n800_tolerance(d, i)
+        report.append((int(idx[i]), S.shape[0], err, float(d["floor"][i]), tol))
         # the device S is as unitary as the oracle's on the same inputs
         assert omatch.unitarity_defect(S) <= 10.0 * float(d["defect"][i]) + 1e-12
-    print(f"N = 800 two-rotor full size: max |dS| = {worst:.3e}, oracle defects {d['defect']}")
-    assert worst <= TOL_S_FULL, worst
+    print("N = 800 two-rotor full size (energy, n_open, max|dS|, floor, tol):", report)
+    for e, n, err, floor, tol in report:
+        assert err <= tol, (e, n, err, floor, tol)
