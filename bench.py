@@ -10,17 +10,22 @@ LDMD sweep of all 64 energies, and the fused BF->SF + K->S matching.
 value = energies x sectors / seconds per step (energy.R-steps/s).
 roofline: the sweep's dominant kernel, k_gj_fused -- the rank-64 Gauss-Jordan
 update of a pair of block steps (FP64 MFMA) with the next pair's panel chain in
-its first workgroups -- timed live with HIP events on the stream it runs on.
-Per launch on m matrices: algorithmic bytes m (2 Np^2 + 64 Np + 128 Np) 8 (the
-update reads X, writes the partner buffer and reads the coefficient columns;
-the panel chain reads its look-ahead columns and writes its coefficients) and
-FP64 flop 2 Np^2 64 m -- about 7.7 flop/byte, below the FP64 ridge point, so
-the bound is HBM; the MFMA rate is reported beside it.  traffic: HBM bytes per
-64-matrix launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-(profiles/), when it was measured on this workload.
+its first workgroups -- timed live with HIP events on the stream it runs on,
+priced against the FP64 matrix peak as the LDMD sweep's accounting does (4 N^3
+FP64 flop per energy.R-step, two N x N inversions): per launch on m matrices
+the update does 2 Np^2 64 m FP64 flop.  Its algorithmic bytes, m (2 Np^2 + 64 Np
++ 128 Np) 8 (the update reads X, writes the partner buffer and reads the
+coefficient columns; the panel chain reads its look-ahead columns and writes
+its coefficients), and the achieved GB/s are reported beside it.  traffic: HBM
+bytes per 64-matrix launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+summary (profiles/), when it was measured on this workload.
+sweep: the whole step's credited rate, 4 N^3 flop per energy.R-step, against
+the same peak.
+parity: the timed run's S at the energies of the committed N = 800 fixture
+(tests/golden/smat_rotor_rotor_N800.npz, written by the oracle) against it.
 cpu_baseline: the NumPy oracle (a restatement of the reference algorithm) on
-the same system, all 64 energies over the first sectors until >= 10 s of
-work, on this host's cores.
+the same system, all 64 energies over the first sectors, about 30 s of work in
+three runs, on this host's cores (CPU model and BLAS library stated).
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu-baseline]
        [--max-sectors M]   (first M sectors only: a short run for counter profiles;
@@ -53,7 +58,20 @@ def _log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(workload, grid, min_seconds: float = 10.0, max_sectors: int = 64, repeats: int = 3) -> dict:
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(workload, grid, min_seconds: float = 30.0, max_sectors: int = 64, repeats: int = 3) -> dict:
     """Time the oracle LDMD (contraction + sweep) on the host for a bounded sample:
     the first sectors from the initial condition, as many as take about
     min_seconds / repeats, run `repeats` times; the median rate is reported."""
@@ -68,6 +86,8 @@ def cpu_baseline(workload, grid, min_seconds: float = 10.0, max_sectors: int = 6
     hs = np.asarray([s.radial_half_step for s in grid.sectors])
     info = threadpoolctl.threadpool_info()
     threads = max((d.get("num_threads", 1) for d in info), default=1)
+    blas = ", ".join(sorted({f"{d.get('internal_api', '?')} {d.get('version', '')}".strip() for d in info
+                             if d.get("user_api") == "blas"})) or "unknown"
     shift = 2 * prob.mu * E
 
     def run(n_max, budget):
@@ -94,7 +114,9 @@ def cpu_baseline(workload, grid, min_seconds: float = 10.0, max_sectors: int = 6
         "kind": "port",
         "sample": f"NumPy oracle LDMD incl. W contraction, N={prob.n_channel}, {E.size} energies x first {n} sectors, "
         f"median of {repeats} runs ({', '.join(f'{t:.1f}' for t in times)} s); host cpus={len(os.sched_getaffinity(0))}, "
-        f"BLAS threads={threads}",
+        f"BLAS threads={threads}; CPU {_cpu_model()}; BLAS {blas}",
+        "cpu_model": _cpu_model(),
+        "blas": blas,
     }
 
 
@@ -196,6 +218,9 @@ def main() -> int:
     defect = max(float(np.max(np.abs(s.conj().T @ s - np.eye(s.shape[0])))) for s in S if s.size)
     _log(f"fused kernel: {n_timed} timed launches, avg {avg_ms * 1e3:.1f} us; unitarity defect {defect:.2e}")
 
+    parity = None if args.max_sectors > 0 else _fixture_parity(S, w)
+    if parity is not None:
+        _log(f"parity vs the N = 800 fixture: max|dS| {parity['max_abs_dS']:.2e} (tolerance {parity['max_tolerance']:.1e})")
     traffic = _traffic(N, nE)
     cpu = None
     if not args.no_cpu_baseline:
@@ -227,25 +252,60 @@ def main() -> int:
         "roofline": {
             "kernel": "k_gj_fused<2> (rank-64 Gauss-Jordan update of a pair of block steps, FP64 MFMA, beside the "
             "next pair's panel chain)",
-            # 7.7 flop per algorithmic byte, below the MI355X FP64 ridge point (78.6 TF / 8 TB/s = 9.8)
-            "bound": "hbm",
-            "achieved": achieved_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": None if achieved_gbs is None else achieved_gbs / HBM_PEAK_GBS,
+            "bound": "mfma",
+            "achieved": achieved_tf,
+            "peak": FP64_MFMA_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": None if achieved_tf is None else achieved_tf / FP64_MFMA_PEAK_TFLOPS,
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "flop_per_launch": flop_per_launch,
             "avg_launch_us": avg_ms * 1e3,
             "timed_launches": n_timed,
-            "mfma_tflops": achieved_tf,
-            "mfma_frac_of_fp64_peak": None if achieved_tf is None else achieved_tf / FP64_MFMA_PEAK_TFLOPS,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "hbm_gbs": achieved_gbs,
+            "hbm_frac": None if achieved_gbs is None else achieved_gbs / HBM_PEAK_GBS,
         },
-        "sweep_tflops_credited": 4.0 * N**3 * nE * nS / (elapsed / args.steps) / 1e12,
+        "sweep": {
+            "credited_flop_per_energy_R_step": 4.0 * N**3,
+            "tflops_credited": 4.0 * N**3 * nE * nS / (elapsed / args.steps) / 1e12,
+            "frac_of_fp64_peak": 4.0 * N**3 * nE * nS / (elapsed / args.steps) / 1e12 / FP64_MFMA_PEAK_TFLOPS,
+        },
         "max_unitarity_defect": defect,
+        "parity": parity,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _fixture_parity(S, workload):
+    """max |dS| of the timed run's S at the energies of the committed N = 800 oracle
+    fixture, each against its per-energy tolerance (the 1e-10 bar or twice the FP64
+    noise floor recorded with the fixture)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "smat_rotor_rotor_N800.npz")
+    try:
+        d = np.load(path)
+    except OSError:
+        return None
+    from tests.golden.fixtures import n800_tolerance, rotor_rotor_N800_reference
+
+    if not np.allclose(d["energies"], workload.energies[d["energy_index"]], rtol=0, atol=0):
+        return None
+    worst, worst_tol, per = 0.0, 0.0, []
+    for i, e in enumerate(d["energy_index"]):
+        s = S[int(e)]
+        if s.shape[0] != int(d["n_open"][i]):
+            return {"error": f"n_open mismatch at energy {int(e)}"}
+        if s.size == 0:
+            continue
+        r, c, ref = rotor_rotor_N800_reference(d, i)
+        err = float(np.max(np.abs(s[r, c] - ref)))
+        tol = n800_tolerance(d, i)
+        per.append({"energy_index": int(e), "n_open": int(s.shape[0]), "max_abs_dS": err, "tolerance": tol})
+        worst = max(worst, err)
+        worst_tol = max(worst_tol, tol)
+    return {"max_abs_dS": worst, "max_tolerance": worst_tol, "ok": all(p["max_abs_dS"] <= p["tolerance"] for p in per),
+            "per_energy": per, "reference": "oracle S at full size (tests/golden/smat_rotor_rotor_N800.npz)"}
 
 
 def bench_batching(args) -> int:

@@ -5,6 +5,7 @@ from __future__ import annotations
 import numpy as np
 
 from oracle.pipeline import Problem
+from tests.golden.fixtures import n800_tolerance, rotor_rotor_N800_reference  # noqa: F401
 from pyticc_amd import prepare_potential
 from pyticc_amd.matrix.centrifugal import get_Umat_BF, ladder_labels
 from pyticc_amd.scattering.hamiltonian import build_hamiltonian
@@ -46,23 +47,3 @@ def random_W_batch(rng, n_sector, N, scale=1.0, diag_shift=0.0):
         out.append(base[None] + 0.5 * (d + np.swapaxes(d, 1, 2)))
     return out
 
-
-def rotor_rotor_N800_reference(d, i):
-    """(rows, cols, values) of the fixture's stored elements of S at its i-th energy."""
-    n = int(d["n_open"][i])
-    if f"S{i}_upper" in d.files:
-        r, c = np.triu_indices(n)
-        return r, c, d[f"S{i}_upper"]
-    if f"S{i}_cols" in d.files:
-        cols = d[f"S{i}_colidx"]
-        r = np.concatenate([np.repeat(np.arange(n), cols.size), np.arange(n)])
-        c = np.concatenate([np.tile(cols, n), np.arange(n)])
-        v = np.concatenate([d[f"S{i}_cols"].ravel(), d[f"S{i}_diag"]])
-        return r, c, v
-    return np.zeros(0, int), np.zeros(0, int), np.zeros(0, complex)
-
-
-def n800_tolerance(d, i, bar=1.0e-10):
-    """Per-energy |dS| tolerance at full size: the 1e-10 bar, or twice the FP64 noise
-    floor recorded with the fixture where that floor exceeds half of the bar."""
-    return max(bar, 2.0 * float(d["floor"][i]))

@@ -31,7 +31,12 @@ def test_headline_bench_line(gpu_device):
     assert d["value"] > 0 and d["n_gpus"] == 1 and d["dtype"] == "f64"
     assert d["config"]["n_channel"] == 800 and d["config"]["n_energy"] == 64 and d["config"]["n_sector"] == 4
     r = d["roofline"]
-    assert r["bound"] == "hbm" and r["unit"] == "GB/s" and r["achieved"] > 0 and 0 < r["frac"] < 1
+    assert r["bound"] == "mfma" and r["unit"] == "TFLOP/s" and r["peak"] == 78.6
+    assert r["achieved"] > 0 and 0 < r["frac"] < 1
+    assert abs(r["achieved"] - r["flop_per_launch"] / (r["avg_launch_us"] * 1e-6) / 1e12) < 1e-6 * r["achieved"]
+    assert 0 < r["hbm_frac"] < 1
+    assert 0 < d["sweep"]["frac_of_fp64_peak"] < 1
+    assert d["parity"] is None  # a reduced run names no fixture comparison
     assert d["max_unitarity_defect"] < 1e-8
 
 
