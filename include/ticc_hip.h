@@ -206,6 +206,23 @@ int ticc_bessel_eval(int n, const double* ell, const double* x, double* out, con
  */
 int ticc_invert_batched(int n_batch, int n_channel, double* A, void* stream);
 
+/* Size of the fixed scratch array of ticc_invert_batched_noalloc, per GPU. */
+#define TICC_NOALLOC_SCRATCH_BYTES (32 * 1024 * 1024)
+
+/*
+ * ticc_invert_batched without any allocation (no counterpart in the reference:
+ * for callers that cannot allocate device memory -- code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * arithmetic and results as ticc_invert_batched, but its scratch (padded
+ * copies, ping-pong partners, panels and row maps) lives in a fixed device array
+ * of TICC_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a call
+ * whose scratch would exceed it (about 12 x Np^2 bytes per matrix plus panels,
+ * Np = N rounded up to 32) returns TICC_ERR_VALUE.  Calls that overlap on one GPU
+ * share that array and race; calls on different GPUs do not.  Synchronises
+ * `stream`.
+ */
+int ticc_invert_batched_noalloc(int n_batch, int n_channel, double* A, void* stream);
+
 /* Batched C = alpha A B + beta C for (m, k) x (k, n) row-major device matrices
  * with batch strides (elements); stride 0 broadcasts.  (Test and utility entry.) */
 int ticc_dgemm_batched(int m, int n, int k, double alpha, const double* A, long long strideA, const double* B,

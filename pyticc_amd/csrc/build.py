@@ -18,7 +18,7 @@ PKG = HERE.parent
 REPO = PKG.parent
 OUT_DIR = PKG / "_lib"
 LIB = OUT_DIR / "libticc_hip.so"
-SOURCES = ("util.hip", "gj.hip", "gemm.hip", "ldmd.hip", "contract.hip", "match.hip", "mlp.hip", "capi.hip")
+SOURCES = ("util.hip", "gj.hip", "gemm.hip", "ldmd.hip", "contract.hip", "match.hip", "mlp.hip", "noalloc.hip", "capi.hip")
 ARCH = os.environ.get("TICC_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -39,6 +39,57 @@ int bessel_eval(int n, const double* ell, const double* x, double* out, const do
 
 using ticc::fail;
 
+namespace ticc {
+namespace {
+// Scratch layout of one batched inversion: the padded matrices and their
+// ping-pong partners, the Gauss-Jordan panels, row maps and pivot lists, and the
+// row permutation and its inverse, each 256-byte aligned.
+struct InvertLayout {
+  size_t o0, o1, oP, oS, oV, oM, oI, total;
+};
+InvertLayout invert_layout(int n_batch, int Np) {
+  InvertLayout l{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t n2 = (size_t)Np * Np;
+  l.o0 = take(sizeof(double) * n_batch * n2);
+  l.o1 = take(sizeof(double) * n_batch * n2);
+  l.oP = take(sizeof(double) * GJ_POUT_PANELS * n_batch * Np * GJ_NB);
+  l.oS = take(sizeof(int) * GJ_SRC_MAPS * n_batch * Np);
+  l.oV = take(sizeof(int) * GJ_PIV_LISTS * n_batch * GJ_NB);
+  l.oM = take(sizeof(int) * n_batch * Np);
+  l.oI = take(sizeof(int) * n_batch * Np);
+  l.total = off;
+  return l;
+}
+}  // namespace
+
+size_t invert_scratch_bytes(int n_batch, int Np) { return invert_layout(n_batch, Np).total; }
+
+int invert_with_scratch(int n_batch, int n_channel, double* A, void* scratch, hipStream_t st) {
+  const int Np = round_up(n_channel, GJ_NB);
+  const InvertLayout l = invert_layout(n_batch, Np);
+  char* w = static_cast<char*>(scratch);
+  double* A0 = reinterpret_cast<double*>(w + l.o0);
+  double* A1 = reinterpret_cast<double*>(w + l.o1);
+  TICC_RETURN_IF(pad_identity(A, (long long)n_channel * n_channel, n_channel, A0, Np, n_batch, st));
+  double* res = nullptr;
+  int* pinv = reinterpret_cast<int*>(w + l.oI);
+  TICC_RETURN_IF(gj_invert_raw(A0, A1, n_batch, Np, &res, reinterpret_cast<int*>(w + l.oM), pinv,
+                               reinterpret_cast<double*>(w + l.oP), reinterpret_cast<int*>(w + l.oS),
+                               reinterpret_cast<int*>(w + l.oV), st));
+  double* other = (res == A0) ? A1 : A0;
+  TICC_RETURN_IF(gj_unpermute(res, other, n_batch, Np, pinv, st));
+  TICC_RETURN_IF(unpad(other, Np, A, (long long)n_channel * n_channel, n_channel, n_batch, st));
+  TICC_CHECK_HIP(hipStreamSynchronize(st));
+  return TICC_OK;
+}
+}  // namespace ticc
+
 namespace {
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
@@ -193,36 +244,10 @@ int ticc_bessel_eval(int n, const double* ell, const double* x, double* out, con
 int ticc_invert_batched(int n_batch, int n_channel, double* A, void* stream) {
   TICC_RETURN_IF(check_channels(n_channel));
   if (n_batch < 1 || !A) return fail(TICC_ERR_VALUE, "invert_batched: invalid arguments");
-  hipStream_t st = as_stream(stream);
-  const int Np = ticc::round_up(n_channel, ticc::GJ_NB);
-  const size_t n2 = (size_t)Np * Np;
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    size_t o = off;
-    off += (bytes + 255) / 256 * 256;
-    return o;
-  };
-  const size_t o0 = take(sizeof(double) * n_batch * n2), o1 = take(sizeof(double) * n_batch * n2);
-  const size_t oP = take(sizeof(double) * ticc::GJ_POUT_PANELS * n_batch * Np * ticc::GJ_NB);
-  const size_t oS = take(sizeof(int) * ticc::GJ_SRC_MAPS * n_batch * Np),
-               oV = take(sizeof(int) * ticc::GJ_PIV_LISTS * n_batch * ticc::GJ_NB);
-  const size_t oM = take(sizeof(int) * n_batch * Np), oI = take(sizeof(int) * n_batch * Np);
+  const size_t bytes = ticc::invert_scratch_bytes(n_batch, ticc::round_up(n_channel, ticc::GJ_NB));
   void* base = nullptr;
-  TICC_RETURN_IF(ticc::workspace(ticc::WS_INVERT, off, &base));
-  char* w = static_cast<char*>(base);
-  double* A0 = reinterpret_cast<double*>(w + o0);
-  double* A1 = reinterpret_cast<double*>(w + o1);
-  TICC_RETURN_IF(ticc::pad_identity(A, (long long)n_channel * n_channel, n_channel, A0, Np, n_batch, st));
-  double* res = nullptr;
-  int* pinv = reinterpret_cast<int*>(w + oI);
-  TICC_RETURN_IF(ticc::gj_invert_raw(A0, A1, n_batch, Np, &res, reinterpret_cast<int*>(w + oM), pinv,
-                                     reinterpret_cast<double*>(w + oP), reinterpret_cast<int*>(w + oS),
-                                     reinterpret_cast<int*>(w + oV), st));
-  double* other = (res == A0) ? A1 : A0;
-  TICC_RETURN_IF(ticc::gj_unpermute(res, other, n_batch, Np, pinv, st));
-  TICC_RETURN_IF(ticc::unpad(other, Np, A, (long long)n_channel * n_channel, n_channel, n_batch, st));
-  TICC_CHECK_HIP(hipStreamSynchronize(st));
-  return TICC_OK;
+  TICC_RETURN_IF(ticc::workspace(ticc::WS_INVERT, bytes, &base));
+  return ticc::invert_with_scratch(n_batch, n_channel, A, base, as_stream(stream));
 }
 
 int ticc_dgemm_batched(int m, int n, int k, double alpha, const double* A, long long strideA, const double* B,

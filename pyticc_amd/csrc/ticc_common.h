@@ -82,6 +82,11 @@ int dgemm_batched(bool transA, bool transB, int m, int n, int k, double alpha, c
 // Identity padding: dst (batch, Np, Np) <- src (batch, N, N) with ld N, identity in the padding.
 int pad_identity(const double* src, long long src_stride, int N, double* dst, int Np, int batch, hipStream_t stream);
 int unpad(const double* src, int Np, double* dst, long long dst_stride, int N, int batch, hipStream_t stream);
+// Batched inversion of n_batch N x N matrices in place (pad, Gauss-Jordan, column
+// unpermute, unpad), its scratch carved from `scratch` (invert_scratch_bytes of
+// it); synchronises `st`.  Shared by the workspace and the no-allocation entries.
+size_t invert_scratch_bytes(int n_batch, int Np);
+int invert_with_scratch(int n_batch, int n_channel, double* A, void* scratch, hipStream_t st);
 // Non-finite scan: *flag (device int) set to 1 if any of n doubles is NaN/Inf.
 int check_finite(const double* x, long long n, int* flag, hipStream_t stream);
 }  // namespace ticc

@@ -35,6 +35,11 @@ def test_invalid_arguments_map_to_value_error():
         _backend.call("ticc_invert_batched", 1, 0, None, None)
     with pytest.raises(NotImplementedError):
         _backend.call("ticc_invert_batched", 1, 5000, None, None)
+    # the no-allocation inversion checks its fixed scratch size before any device call
+    with pytest.raises(ValueError, match="scratch"):
+        _backend.call("ticc_invert_batched_noalloc", 64, 800, None, None)
+    with pytest.raises(ValueError, match="n_channel"):
+        _backend.call("ticc_invert_batched_noalloc", 1, 0, None, None)
 
 
 def test_lane_and_walk_knobs_validate_without_gpu():
