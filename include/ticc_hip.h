@@ -142,6 +142,21 @@ int ticc_ldmd_propagate_blocks(int n_block, int n_energy, int n_channel, const d
                                void* stream);
 
 /*
+ * Mixed-precision LDMD sweep (the precision form; no counterpart in the
+ * reference, whose sweep is jnp FP64 throughout, src/pyticc/propagation/logd.py:168-194).
+ * Same arguments and results as ticc_ldmd_propagate, but each sector's two
+ * inversions run as an FP32 Gauss-Jordan inversion (FP32 panels and FP32 MFMA
+ * updates, the same pivot rule) followed by refine_steps (0..3) Newton steps
+ * X <- X + X (I - A X): the residual I - A X from the FP64 A on the FP64 MFMA,
+ * the correction on the FP32 MFMA.  Reference values, corrections, Q_mid, the
+ * form steps and Y stay FP64.  Two steps keep the S matrix within 1e-6 of the
+ * FP64 sweep on the benchmark systems.
+ */
+int ticc_ldmd_propagate_mixed(int n_energy, int n_channel, const double* energies, double reduced_mass, int n_sector,
+                              const double* half_steps, const double* W_start, const double* W_mid, const double* W_end,
+                              long long w_sector_stride, int refine_steps, double* Y, void* stream);
+
+/*
  * Capture boundary condition (Propagation(mode="capture")): complex symmetric
  * log derivatives Z carried in the real form [[Re Z, -Im Z], [Im Z, Re Z]]
  * (2N x 2N), on which sums, products, inverses and real diagonal shifts act as
@@ -222,6 +237,14 @@ int ticc_invert_batched(int n_batch, int n_channel, double* A, void* stream);
  * `stream`.
  */
 int ticc_invert_batched_noalloc(int n_batch, int n_channel, double* A, void* stream);
+
+/*
+ * Batched in-place inverse by the mixed-precision scheme of
+ * ticc_ldmd_propagate_mixed: FP32 Gauss-Jordan with partial pivoting, then
+ * refine_steps (0..3) Newton steps with the residual from the FP64 input.
+ * (Test and utility entry; no counterpart in the reference.)
+ */
+int ticc_invert_batched_mixed(int n_batch, int n_channel, int refine_steps, double* A, void* stream);
 
 /* Batched C = alpha A B + beta C for (m, k) x (k, n) row-major device matrices
  * with batch strides (elements); stride 0 broadcasts.  (Test and utility entry.) */

@@ -53,6 +53,11 @@ _SIGNATURES = {
     "ticc_get_smat": ([_c_int, _c_int, _vp, _vp, _c_d, _dp, _c_d, _dp, _dp, _vp, _vp, _ip, _vp], _c_int),
     "ticc_bessel_eval": ([_c_int, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "ticc_invert_batched": ([_c_int, _c_int, _vp, _vp], _c_int),
+    "ticc_invert_batched_mixed": ([_c_int, _c_int, _c_int, _vp, _vp], _c_int),
+    "ticc_ldmd_propagate_mixed": (
+        [_c_int, _c_int, _dp, _c_d, _c_int, _dp, _vp, _vp, _vp, _c_ll, _c_int, _vp, _vp],
+        _c_int,
+    ),
     "ticc_invert_batched_noalloc": ([_c_int, _c_int, _vp, _vp], _c_int),
     "ticc_set_sweep_walk": ([_c_int, _c_int], _c_int),
     "ticc_profile_update_begin": ([_c_int, _c_int], _c_int),

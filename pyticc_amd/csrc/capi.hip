@@ -26,7 +26,7 @@ int mlp_adam(int rows, int cols, float* theta, const float* grad, float* m, floa
 int mlp_to_bf16(int rows, int cols, const float* x, void* bf, void* bfT, hipStream_t stream);
 int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const double* hs_host, const double* Ws0,
                    const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream, int half,
-                   int n_block, long long gstride);
+                   int n_block, long long gstride, int refine);
 int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, const double* E_host, double mu,
              const double* Eint_host, const double* L_host, double* S_re, double* S_im, int* n_open,
              hipStream_t stream);
@@ -186,7 +186,7 @@ int ticc_ldmd_propagate_capture(int n_energy, int n_channel, const double* energ
   TICC_RETURN_IF(sweep_args(n_energy, n_sector, energies, half_steps, W_start, W_mid, W_end, Y));
   if (n_sector == 0) return TICC_OK;
   return ticc::ldmd_propagate(n_energy, 2 * n_channel, energies, reduced_mass, n_sector, half_steps, W_start, W_mid,
-                              W_end, w_sector_stride, Y, as_stream(stream), n_channel, 1, 0);
+                              W_end, w_sector_stride, Y, as_stream(stream), n_channel, 1, 0, -1);
 }
 
 int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, double reduced_mass, int n_sector,
@@ -196,7 +196,19 @@ int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, dou
   TICC_RETURN_IF(sweep_args(n_energy, n_sector, energies, half_steps, W_start, W_mid, W_end, Y));
   if (n_sector == 0) return TICC_OK;
   return ticc::ldmd_propagate(n_energy, n_channel, energies, reduced_mass, n_sector, half_steps, W_start, W_mid, W_end,
-                              w_sector_stride, Y, as_stream(stream), 0, 1, 0);
+                              w_sector_stride, Y, as_stream(stream), 0, 1, 0, -1);
+}
+
+int ticc_ldmd_propagate_mixed(int n_energy, int n_channel, const double* energies, double reduced_mass, int n_sector,
+                              const double* half_steps, const double* W_start, const double* W_mid, const double* W_end,
+                              long long w_sector_stride, int refine_steps, double* Y, void* stream) {
+  TICC_RETURN_IF(check_channels(n_channel));
+  TICC_RETURN_IF(sweep_args(n_energy, n_sector, energies, half_steps, W_start, W_mid, W_end, Y));
+  if (refine_steps < 0 || refine_steps > 3)
+    return fail(TICC_ERR_VALUE, "refine_steps must lie in [0, 3], got %d", refine_steps);
+  if (n_sector == 0) return TICC_OK;
+  return ticc::ldmd_propagate(n_energy, n_channel, energies, reduced_mass, n_sector, half_steps, W_start, W_mid, W_end,
+                              w_sector_stride, Y, as_stream(stream), 0, 1, 0, refine_steps);
 }
 
 int ticc_ldmd_propagate_blocks(int n_block, int n_energy, int n_channel, const double* energies, double reduced_mass,
@@ -212,7 +224,7 @@ int ticc_ldmd_propagate_blocks(int n_block, int n_energy, int n_channel, const d
   for (int b = 0; b < n_block; ++b)
     for (int e = 0; e < n_energy; ++e) all[(size_t)b * n_energy + e] = energies[e];
   return ticc::ldmd_propagate(n_block * n_energy, n_channel, all.data(), reduced_mass, n_sector, half_steps, W_start,
-                              W_mid, W_end, w_sector_stride, Y, as_stream(stream), 0, n_block, w_block_stride);
+                              W_mid, W_end, w_sector_stride, Y, as_stream(stream), 0, n_block, w_block_stride, -1);
 }
 
 int ticc_get_smat(int n_energy, int n_channel, const double* Y, const double* B, double Rmatch,
@@ -248,6 +260,51 @@ int ticc_invert_batched(int n_batch, int n_channel, double* A, void* stream) {
   void* base = nullptr;
   TICC_RETURN_IF(ticc::workspace(ticc::WS_INVERT, bytes, &base));
   return ticc::invert_with_scratch(n_batch, n_channel, A, base, as_stream(stream));
+}
+
+int ticc_invert_batched_mixed(int n_batch, int n_channel, int refine_steps, double* A, void* stream) {
+  TICC_RETURN_IF(check_channels(n_channel));
+  if (n_batch < 1 || !A) return fail(TICC_ERR_VALUE, "invert_batched_mixed: invalid arguments");
+  if (refine_steps < 0 || refine_steps > 3)
+    return fail(TICC_ERR_VALUE, "refine_steps must lie in [0, 3], got %d", refine_steps);
+  hipStream_t st = as_stream(stream);
+  const int Np = ticc::round_up(n_channel, ticc::GJ_NB);
+  const size_t n2 = (size_t)Np * Np;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t oA = take(sizeof(double) * n_batch * n2), oX = take(sizeof(double) * n_batch * n2);
+  const size_t oF0 = take(sizeof(float) * n_batch * n2), oF1 = take(sizeof(float) * n_batch * n2);
+  const size_t oT = take(sizeof(float) * n_batch * n2), oX0 = take(sizeof(float) * n_batch * n2),
+               oX1 = take(sizeof(float) * n_batch * n2);
+  const size_t oP = take(sizeof(float) * ticc::GJ_POUT_PANELS * n_batch * Np * ticc::GJ_NB);
+  const size_t oS = take(sizeof(int) * ticc::GJ_SRC_MAPS * n_batch * Np),
+               oV = take(sizeof(int) * ticc::GJ_PIV_LISTS * n_batch * ticc::GJ_NB);
+  const size_t oM = take(sizeof(int) * n_batch * Np), oI = take(sizeof(int) * n_batch * Np);
+  void* base = nullptr;
+  TICC_RETURN_IF(ticc::workspace(ticc::WS_INVERT, off, &base));
+  char* w = static_cast<char*>(base);
+  auto D = [&](size_t o) { return reinterpret_cast<double*>(w + o); };
+  auto F = [&](size_t o) { return reinterpret_cast<float*>(w + o); };
+  auto I = [&](size_t o) { return reinterpret_cast<int*>(w + o); };
+  TICC_RETURN_IF(ticc::pad_identity(A, (long long)n_channel * n_channel, n_channel, D(oA), Np, n_batch, st));
+  TICC_RETURN_IF(ticc::to_f32(D(oA), F(oF0), (long long)n_batch * n2, st));
+  float* r32 = nullptr;
+  TICC_RETURN_IF(ticc::gj_invert_raw_f32(F(oF0), F(oF1), n_batch, Np, &r32, I(oM), I(oI), F(oP), I(oS), I(oV), st));
+  TICC_RETURN_IF(ticc::gj_unpermute_f32(r32, D(oX), n_batch, Np, I(oI), st));
+  if (refine_steps > 0) TICC_RETURN_IF(ticc::gj_unpermute_f32f32(r32, F(oX0), n_batch, Np, I(oI), st));
+  float* X32[2] = {F(oX0), F(oX1)};
+  for (int it = 0; it < refine_steps; ++it) {
+    TICC_RETURN_IF(ticc::refine_gemm_f64(Np, n_batch, D(oA), D(oX), -1.0, nullptr, 0.0, 1.0, nullptr, F(oT), st));
+    TICC_RETURN_IF(ticc::refine_gemm_f32(Np, n_batch, X32[it & 1], F(oT), 1.0, D(oX), 1.0, 0.0, D(oX),
+                                         it + 1 < refine_steps ? X32[(it + 1) & 1] : nullptr, st));
+  }
+  TICC_RETURN_IF(ticc::unpad(D(oX), Np, A, (long long)n_channel * n_channel, n_channel, n_batch, st));
+  TICC_CHECK_HIP(hipStreamSynchronize(st));
+  return TICC_OK;
 }
 
 int ticc_dgemm_batched(int m, int n, int k, double alpha, const double* A, long long strideA, const double* B,

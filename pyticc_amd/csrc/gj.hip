@@ -123,37 +123,38 @@ __device__ __forceinline__ void wave_argmax_dpp(double& v, int& p) {
 // the column's single barrier every thread picks the global winner, applies
 // the row swap to its own position records and performs the in-place
 // Gauss-Jordan update, streaming the pivot row from LDS.
-template <int THREADS, int RPT>
-__device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long long bstride, int ld, int col0,
-                                              int Np, int k0, double* __restrict__ Pout, int* __restrict__ src,
+template <int THREADS, int RPT, typename T>
+__device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long bstride, int ld, int col0,
+                                              int Np, int k0, T* __restrict__ Pout, int* __restrict__ src,
                                               int* __restrict__ piv, int* __restrict__ perm, int* __restrict__ pinv,
-                                              double* __restrict__ PoutT, long long tstride,
+                                              T* __restrict__ PoutT, long long tstride,
                                               const int* __restrict__ prev_src, const int* __restrict__ prev_piv,
                                               int prev_k0, int* __restrict__ src2, int* __restrict__ brow, int b) {
+  typedef Elem<T> E;
   constexpr int NW = THREADS / 64;
-  __shared__ double s_vals[2][NW][RPT][NB];
-  __shared__ double s_inv[NB];
+  __shared__ T s_vals[2][NW][RPT][NB];
+  __shared__ T s_inv[NB];
   __shared__ unsigned long long s_key[2][NW];
   __shared__ int s_pos[2][NW];
   __shared__ int s_row[2][NW];
-  __shared__ double s_pinv[2][NW];
-  __shared__ double s_pnext[2][NW];
+  __shared__ T s_pinv[2][NW];
+  __shared__ T s_pnext[2][NW];
 
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = t >> 6;
-  const double* Xb = X + (size_t)b * bstride;
+  const T* Xb = X + (size_t)b * bstride;
 
-  double v[RPT][NB];
+  T v[RPT][NB];
   int pos[RPT];
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     const int r = t + THREADS * q;
     if (r < Np) {
-      const double2* rp = reinterpret_cast<const double2*>(Xb + (size_t)r * ld + col0);
+      const typename E::vec2* rp = reinterpret_cast<const typename E::vec2*>(Xb + (size_t)r * ld + col0);
 #pragma unroll
       for (int l = 0; l < NB / 2; ++l) {
-        const double2 d = rp[l];
+        const typename E::vec2 d = rp[l];
         v[q][2 * l] = d.x;
         v[q][2 * l + 1] = d.y;
       }
@@ -184,7 +185,7 @@ __device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long
     int cq = -1;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-      double a = fabs(v[q][u]);
+      double a = fabs((double)v[q][u]);
       a = (a != a) ? __builtin_inf() : a;
       const bool take = (pos[q] >= c) && better(a, pos[q], cv, cp);
       cv = take ? a : cv;
@@ -226,7 +227,7 @@ __device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long
     // own LDS slot, so the write needs no register select), its key, position,
     // row, the reciprocal of its pivot value and its value in the next column.
     if (lane == wl) {
-      double pv = 0.0, pn = 0.0;
+      T pv = 0, pn = 0;
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         if (cq == q) {
@@ -239,7 +240,7 @@ __device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long
       s_key[buf][wave] = key;
       s_pos[buf][wave] = cp;
       s_row[buf][wave] = t + THREADS * cq;
-      s_pinv[buf][wave] = 1.0 / pv;
+      s_pinv[buf][wave] = T(1) / pv;
       s_pnext[buf][wave] = pn;
     } else if (wl < 0 && lane == 0) {
       s_key[buf][wave] = 0ull;
@@ -268,9 +269,9 @@ __device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long
     const int bw = ww[0];
     const int bp = pw[0];
     const int pr = s_row[buf][bw];
-    const double inv = s_pinv[buf][bw];
-    const double pn = s_pnext[buf][bw];
-    const double* prow = s_vals[buf][bw][pr / THREADS];
+    const T inv = s_pinv[buf][bw];
+    const T pn = s_pnext[buf][bw];
+    const T* prow = s_vals[buf][bw][pr / THREADS];
     // Gauss-Jordan step with the pivot row's scaling deferred: every other row
     // r takes a_rl -= (a_rj / p) p_l, a_rj = -a_rj / p; the pivot row keeps its
     // values with a_rj = 1.  A row that was a pivot row stays p times its true
@@ -279,32 +280,32 @@ __device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long
     // pivot row's own update is a no-op through g = 0, so no lane branches.
     // The next column is updated first: the next pivot search needs only it.
     if (t == 0) s_inv[j] = inv;
-    double g[RPT];
+    T g[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
       const bool is_piv = (t + THREADS * q == pr);
       if (pos[q] == c) pos[q] = bp;
       pos[q] = is_piv ? c : pos[q];
-      g[q] = is_piv ? 0.0 : v[q][u] * inv;
+      g[q] = is_piv ? T(0) : v[q][u] * inv;
     }
     // (u + 1 is the next column, or for the last column another column of
     // the window; pn is the pivot row's value there either way)
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) v[q][u + 1] = fma(-g[q], pn, v[q][u + 1]);
+    for (int q = 0; q < RPT; ++q) v[q][u + 1] = E::fma(-g[q], pn, v[q][u + 1]);
 #pragma unroll
     for (int l = 0; l < NB; ++l) {
       if (l == u || l == u + 1) continue;
-      const double pl = prow[l];
+      const T pl = prow[l];
 #pragma unroll
-      for (int q = 0; q < RPT; ++q) v[q][l] = fma(-g[q], pl, v[q][l]);
+      for (int q = 0; q < RPT; ++q) v[q][l] = E::fma(-g[q], pl, v[q][l]);
     }
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) v[q][u] = (t + THREADS * q == pr) ? 1.0 : -g[q];
+    for (int q = 0; q < RPT; ++q) v[q][u] = (t + THREADS * q == pr) ? T(1) : -g[q];
     }
     // rotate the window by PU columns
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-      double tmp[PU];
+      T tmp[PU];
 #pragma unroll
       for (int l = 0; l < PU; ++l) tmp[l] = v[q][l];
 #pragma unroll
@@ -319,7 +320,7 @@ __device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     if (pos[q] >= k0 && pos[q] < k0 + NB) {
-      const double ri = s_inv[pos[q] - k0];
+      const T ri = s_inv[pos[q] - k0];
 #pragma unroll
       for (int l = 0; l < NB; ++l) v[q][l] *= ri;
     }
@@ -333,9 +334,9 @@ __device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long
     pold[q] = 0;
     if (pos[q] < 0) continue;
     const int p = pos[q];
-    double2* op = reinterpret_cast<double2*>(Pout + ((size_t)b * Np + p) * NB);
+    typename E::vec2* op = reinterpret_cast<typename E::vec2*>(Pout + ((size_t)b * Np + p) * NB);
 #pragma unroll
-    for (int l = 0; l < NB / 2; ++l) op[l] = make_double2(v[q][2 * l], v[q][2 * l + 1]);
+    for (int l = 0; l < NB / 2; ++l) op[l] = E::pair(v[q][2 * l], v[q][2 * l + 1]);
     src[(size_t)b * Np + p] = r;
     const bool blk = (p >= k0 && p < k0 + NB);
     if (blk) piv[b * NB + (p - k0)] = r;
@@ -365,13 +366,13 @@ __device__ __forceinline__ void gj_panel_body(const double* __restrict__ X, long
   }
 }
 
-template <int THREADS, int RPT>
+template <int THREADS, int RPT, typename T>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 256, THREADS / 256))) void k_gj_panel(
-    const double* __restrict__ X, long long bstride, int ld, int col0, int Np, int k0, double* __restrict__ Pout,
+    const T* __restrict__ X, long long bstride, int ld, int col0, int Np, int k0, T* __restrict__ Pout,
     int* __restrict__ src, int* __restrict__ piv, int* __restrict__ perm, int* __restrict__ pinv,
-    double* __restrict__ PoutT, long long tstride, const int* __restrict__ prev_src, const int* __restrict__ prev_piv,
+    T* __restrict__ PoutT, long long tstride, const int* __restrict__ prev_src, const int* __restrict__ prev_piv,
     int prev_k0, int* __restrict__ src2, int* __restrict__ brow) {
-  gj_panel_body<THREADS, RPT>(X, bstride, ld, col0, Np, k0, Pout, src, piv, perm, pinv, PoutT, tstride, prev_src,
+  gj_panel_body<THREADS, RPT, T>(X, bstride, ld, col0, Np, k0, Pout, src, piv, perm, pinv, PoutT, tstride, prev_src,
                               prev_piv, prev_k0, src2, brow, blockIdx.x);
 }
 
@@ -388,6 +389,27 @@ __device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t r, int 
 __device__ __forceinline__ double2 buffer_load_f64x2(__amdgpu_buffer_rsrc_t r, int off) {
   return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
+// One element / a pair of elements of type T at byte offset `off`.
+template <typename T>
+__device__ __forceinline__ T load1(__amdgpu_buffer_rsrc_t r, int off);
+template <>
+__device__ __forceinline__ double load1<double>(__amdgpu_buffer_rsrc_t r, int off) {
+  return buffer_load_f64(r, off);
+}
+template <>
+__device__ __forceinline__ float load1<float>(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <typename T>
+__device__ __forceinline__ typename Elem<T>::vec2 load2(__amdgpu_buffer_rsrc_t r, int off);
+template <>
+__device__ __forceinline__ double2 load2<double>(__amdgpu_buffer_rsrc_t r, int off) {
+  return buffer_load_f64x2(r, off);
+}
+template <>
+__device__ __forceinline__ float2 load2<float>(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
 
 // Small panel GEMM (FP64 MFMA) of the paired block step, for one matrix b:
 //   Out[i][0:NB] = mask(i) C[gC(i)][colC : colC+NB] + A[i][0:NB] . B[gB(r)][colB : colB+NB],
@@ -398,24 +420,25 @@ __device__ __forceinline__ double2 buffer_load_f64x2(__amdgpu_buffer_rsrc_t r, i
 // operands of GCH blocks are loaded together, so the gathers' latency is paid
 // twice per GCH blocks instead of twice per block.  sB stages the NB gathered
 // B rows.
-template <int THREADS, int MAXROWS>
-__device__ __forceinline__ void panel_gemm_body(const double* __restrict__ A, const double* __restrict__ C,
+template <int THREADS, int MAXROWS, typename T>
+__device__ __forceinline__ void panel_gemm_body(const T* __restrict__ A, const T* __restrict__ C,
                                                 long long cb, int ldC, int colC, const int* __restrict__ gC,
-                                                const double* __restrict__ Bm, long long bb, int ldB, int colB,
+                                                const T* __restrict__ Bm, long long bb, int ldB, int colB,
                                                 const int* __restrict__ gB, int mlo, int Np,
-                                                double* __restrict__ Out, long long ostride, int b,
-                                                double (*sB)[NB + 1]) {
+                                                T* __restrict__ Out, long long ostride, int b,
+                                                T (*sB)[NB + 1]) {
+  typedef Elem<T> E;
   constexpr int RB = THREADS / 4;                 // rows per round of all waves
   constexpr int MAXIT = (MAXROWS + RB - 1) / RB;  // rounds at the largest Np of the caller
   constexpr int GCH = 4;                          // rounds whose operands are in flight together
   static_assert(MAXIT % GCH == 0, "round count must be a multiple of the chunk");
   const int t = threadIdx.x;
   const int wave = t >> 6, lane = t & 63, lr = lane & 15, lq = lane >> 4;
-  const double* Cb = C + (size_t)b * cb;
-  const auto rC = make_rsrc(Cb, (unsigned)(sizeof(double) * Np * ldC));
-  const auto rA = make_rsrc(A + (size_t)b * Np * NB, (unsigned)(sizeof(double) * Np * NB));
+  const T* Cb = C + (size_t)b * cb;
+  const auto rC = make_rsrc(Cb, (unsigned)(sizeof(T) * Np * ldC));
+  const auto rA = make_rsrc(A + (size_t)b * Np * NB, (unsigned)(sizeof(T) * Np * NB));
   const auto rG = make_rsrc(gC + (size_t)b * Np, (unsigned)(sizeof(int) * Np));
-  const double* Bb = Bm + (size_t)b * bb;
+  const T* Bb = Bm + (size_t)b * bb;
   const int* gBb = gB + b * NB;
   for (int e = t; e < NB * NB; e += THREADS) {
     const int r = e / NB, cc = e % NB;
@@ -430,25 +453,25 @@ __device__ __forceinline__ void panel_gemm_body(const double* __restrict__ A, co
     for (int u = 0; u < GCH; ++u)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = (c0 + u) * RB + wave * 16 + mfma_drow(lq, r);
+        const int row = (c0 + u) * RB + wave * 16 + E::drow(lq, r);
         const bool live = row < Np && !(row >= mlo && row < mlo + NB);
         const int g = (int)__builtin_amdgcn_raw_buffer_load_b32(rG, live ? 4 * row : kOutOfRange, 0, 0);
         src[u][r] = live ? g : -1;
       }
-    double4_t acc[GCH][2];
-    double a[GCH][NB / 4];
+    typename E::acc acc[GCH][2];
+    T a[GCH][NB / 4];
 #pragma unroll
     for (int u = 0; u < GCH; ++u) {
       const int row0 = (c0 + u) * RB + wave * 16;
 #pragma unroll
       for (int kk = 0; kk < NB / 4; ++kk)
-        a[u][kk] = buffer_load_f64(rA, row0 + lr < Np ? 8 * ((row0 + lr) * NB + 4 * kk + lq) : kOutOfRange);
+        a[u][kk] = load1<T>(rA, row0 + lr < Np ? (int)sizeof(T) * ((row0 + lr) * NB + 4 * kk + lq) : kOutOfRange);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int s = src[u][r];
-        const int base = s >= 0 ? 8 * (s * ldC + colC + lr) : kOutOfRange;
+        const int base = s >= 0 ? (int)sizeof(T) * (s * ldC + colC + lr) : kOutOfRange;
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) acc[u][bj][r] = buffer_load_f64(rC, base + 128 * bj);
+        for (int bj = 0; bj < 2; ++bj) acc[u][bj][r] = load1<T>(rC, base + 16 * (int)sizeof(T) * bj);
       }
     }
 #pragma unroll
@@ -459,23 +482,23 @@ __device__ __forceinline__ void panel_gemm_body(const double* __restrict__ A, co
         const int kq = 4 * kk + lq;
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj)
-          acc[u][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][kk], sB[kq][16 * bj + lr], acc[u][bj], 0, 0, 0);
+          acc[u][bj] = E::mfma(a[u][kk], sB[kq][16 * bj + lr], acc[u][bj]);
       }
       if (ostride > 0) {
-        double* Ob = Out + (size_t)b * ostride;
+        T* Ob = Out + (size_t)b * ostride;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = row0 + mfma_drow(lq, r);
+          const int row = row0 + E::drow(lq, r);
           if (row < Np) {
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj) Ob[(size_t)(16 * bj + lr) * Np + row] = acc[u][bj][r];
           }
         }
       } else {
-        double* Ob = Out + (size_t)b * Np * NB;
+        T* Ob = Out + (size_t)b * Np * NB;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = row0 + mfma_drow(lq, r);
+          const int row = row0 + E::drow(lq, r);
           if (row < Np) {
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj) Ob[(size_t)row * NB + 16 * bj + lr] = acc[u][bj][r];
@@ -491,43 +514,44 @@ __device__ __forceinline__ void panel_gemm_body(const double* __restrict__ A, co
 // Z = the columns of step k0+NB after step k0; panel of Z (also writing the
 // transposed coefficients and the pair's row maps); the combined step-k0
 // coefficients, written transposed to AT[0:NB].
+template <typename T>
 struct PairJob {
-  const double* Xc;
+  const T* Xc;
   long long xs;
   int ldx, xcol, Np, k0;
   int single;  // 1: only the panel of the trailing single step (with its update maps)
-  double *P1, *P2, *Z, *AT;
+  T *P1, *P2, *Z, *AT;
   int *s1, *s2, *p1, *p2, *src2, *brow, *perm, *pinv;
 };
 
-template <int RPT>
-__device__ __forceinline__ void gj_pair_chain_body(const PairJob& P, int b) {
+template <int RPT, typename T>
+__device__ __forceinline__ void gj_pair_chain_body(const PairJob<T>& P, int b) {
   // row bound of the panel GEMMs: 1024 up to two rows per thread, else MAX_NP
   constexpr int GR = RPT <= 2 ? 1024 : MAX_NP;
-  __shared__ double sG[NB][NB + 1];
+  __shared__ T sG[NB][NB + 1];
   const int Np = P.Np, k0 = P.k0, k1 = P.k0 + NB;
   const long long pn = (long long)Np * NB;
   const long long at_stride = 2LL * NB * Np;
   if (P.single) {
-    gj_panel_body<512, RPT>(P.Xc, P.xs, P.ldx, P.xcol, Np, k0, P.P1, P.s1, P.p1, P.perm, P.pinv, P.AT, at_stride,
+    gj_panel_body<512, RPT, T>(P.Xc, P.xs, P.ldx, P.xcol, Np, k0, P.P1, P.s1, P.p1, P.perm, P.pinv, P.AT, at_stride,
                             nullptr, nullptr, 0, P.src2, P.brow, b);
     return;
   }
-  gj_panel_body<512, RPT>(P.Xc, P.xs, P.ldx, P.xcol, Np, k0, P.P1, P.s1, P.p1, P.perm, P.pinv, nullptr, 0, nullptr,
+  gj_panel_body<512, RPT, T>(P.Xc, P.xs, P.ldx, P.xcol, Np, k0, P.P1, P.s1, P.p1, P.perm, P.pinv, nullptr, 0, nullptr,
                           nullptr, 0, nullptr, nullptr, b);
   __syncthreads();
-  panel_gemm_body<512, GR>(P.P1, P.Xc, P.xs, P.ldx, P.xcol + NB, P.s1, P.Xc, P.xs, P.ldx, P.xcol + NB, P.p1, k0, Np, P.Z,
+  panel_gemm_body<512, GR, T>(P.P1, P.Xc, P.xs, P.ldx, P.xcol + NB, P.s1, P.Xc, P.xs, P.ldx, P.xcol + NB, P.p1, k0, Np, P.Z,
                        0LL, b, sG);
   __syncthreads();
-  gj_panel_body<512, RPT>(P.Z, pn, NB, 0, Np, k1, P.P2, P.s2, P.p2, P.perm, P.pinv, P.AT + (size_t)NB * Np, at_stride,
+  gj_panel_body<512, RPT, T>(P.Z, pn, NB, 0, Np, k1, P.P2, P.s2, P.p2, P.perm, P.pinv, P.AT + (size_t)NB * Np, at_stride,
                           P.s1, P.p1, k0, P.src2, P.brow, b);
   __syncthreads();
-  panel_gemm_body<512, GR>(P.P2, P.P1, pn, NB, 0, P.s2, P.P1, pn, NB, 0, P.p2, k1, Np, P.AT, at_stride, b, sG);
+  panel_gemm_body<512, GR, T>(P.P2, P.P1, pn, NB, 0, P.s2, P.P1, pn, NB, 0, P.p2, k1, Np, P.AT, at_stride, b, sG);
 }
 
-template <int RPT>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_pair_chain(PairJob P) {
-  gj_pair_chain_body<RPT>(P, blockIdx.x);
+template <int RPT, typename T>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_pair_chain(PairJob<T> P) {
+  gj_pair_chain_body<RPT, T>(P, blockIdx.x);
 }
 
 // Rank-KB update (KB = NB for one step, 2 NB for a pair of steps):
@@ -543,12 +567,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // registers, and the next tile's C rows (into the idle accumulator set) and B
 // rows (into the idle LDS buffer) are loaded while the current tile's MFMAs
 // and stores run.
-template <int KB, int NBANDS, bool ONE = false>
-__device__ __forceinline__ void gj_update_body(const double* __restrict__ X, double* __restrict__ Out,
+template <int KB, int NBANDS, bool ONE, typename T>
+__device__ __forceinline__ void gj_update_body(const T* __restrict__ X, T* __restrict__ Out,
                                                long long ostride, int ldo, int ocol0, int Np, int k0,
-                                               const double* __restrict__ AT, const int* __restrict__ src2,
+                                               const T* __restrict__ AT, const int* __restrict__ src2,
                                                const int* __restrict__ brow, int b, int band0, int jt0, int jt1,
-                                               int t, double (*sB)[KB][64]) {
+                                               int t, T (*sB)[KB][64]) {
+  typedef Elem<T> E;
+  constexpr int SZ = (int)sizeof(T);
   constexpr int KK = KB / 4;
   constexpr int RP = 8 * NBANDS;  // B-tile rows staged per pass by the whole workgroup
   const int th = t & 255;         // thread within its band's 4 waves
@@ -557,26 +583,26 @@ __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, dou
   const int i0 = band * 64 + (wave >> 1) * 32;
   const int jw = (wave & 1) * 32;
   const bool rows_ok = i0 < Np;
-  const double* Xb = X + (size_t)b * Np * Np;
-  double* Ob = Out + (size_t)b * ostride - ocol0;
-  const auto rX = make_rsrc(Xb, (unsigned)(sizeof(double) * Np * Np));
-  const auto rA = make_rsrc(AT + (size_t)b * 2 * NB * Np, (unsigned)(sizeof(double) * KB * Np));
+  const T* Xb = X + (size_t)b * Np * Np;
+  T* Ob = Out + (size_t)b * ostride - ocol0;
+  const auto rX = make_rsrc(Xb, (unsigned)(sizeof(T) * Np * Np));
+  const auto rA = make_rsrc(AT + (size_t)b * 2 * NB * Np, (unsigned)(sizeof(T) * KB * Np));
   const auto rS = make_rsrc(src2 + (size_t)b * Np, (unsigned)(sizeof(int) * Np));
   const int* brb = brow + b * 2 * NB;
 
-  double a[2][KK];
+  T a[2][KK];
   int sr[2][4];
 #pragma unroll
   for (int kk = 0; kk < KK; ++kk)
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
-      a[bi][kk] = buffer_load_f64(rA, rows_ok ? 8 * ((4 * kk + lq) * Np + i0 + 16 * bi + lr) : kOutOfRange);
+      a[bi][kk] = load1<T>(rA, rows_ok ? SZ * ((4 * kk + lq) * Np + i0 + 16 * bi + lr) : kOutOfRange);
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int v = (int)__builtin_amdgcn_raw_buffer_load_b32(
-          rS, rows_ok ? 4 * (i0 + 16 * bi + mfma_drow(lq, r)) : kOutOfRange, 0, 0);
+          rS, rows_ok ? 4 * (i0 + 16 * bi + E::drow(lq, r)) : kOutOfRange, 0, 0);
       sr[bi][r] = rows_ok ? v : -1;
     }
   const int bk = t >> 5, c2 = (t & 31) * 2;
@@ -593,7 +619,7 @@ __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, dou
   // live = false: every load takes the out-of-range offset (reads 0, no
   // memory access).  The prefetches are issued on every path so the compiler's
   // wait counts before the current tile's MFMAs need not cover them.
-  auto load_c = [&](int J, bool live, double4_t(&c)[2][2]) {
+  auto load_c = [&](int J, bool live, typename E::acc(&c)[2][2]) {
     const int j0 = J * 64 + jw;
     const bool upd = live && kind(J) == 0;
 #pragma unroll
@@ -601,17 +627,17 @@ __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, dou
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int s = sr[bi][r];
-        const int base = (upd && s >= 0) ? 8 * (s * Np + j0 + lr) : kOutOfRange;
+        const int base = (upd && s >= 0) ? SZ * (s * Np + j0 + lr) : kOutOfRange;
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) c[bi][bj][r] = buffer_load_f64(rX, base + 128 * bj);
+        for (int bj = 0; bj < 2; ++bj) c[bi][bj][r] = load1<T>(rX, base + 16 * SZ * bj);
       }
   };
-  double2 bst[KB / RP];
+  typename E::vec2 bst[KB / RP];
   auto load_b = [&](int J, bool live) {
     const bool in = live && (J * 64 + c2) < Np;
 #pragma unroll
     for (int h = 0; h < KB / RP; ++h)
-      bst[h] = buffer_load_f64x2(rX, in ? 8 * (bro[h] * Np + J * 64 + c2) : kOutOfRange);
+      bst[h] = load2<T>(rX, in ? SZ * (bro[h] * Np + J * 64 + c2) : kOutOfRange);
   };
   // B tile rows are stored with bit 4 of the column flipped on odd rows, so the
   // four rows an MFMA B fragment reads fall in different LDS bank halves.
@@ -619,10 +645,10 @@ __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, dou
 #pragma unroll
     for (int h = 0; h < KB / RP; ++h) {
       const int k = h * RP + bk;
-      *reinterpret_cast<double2*>(&sB[buf][k][c2 ^ ((k & 1) << 4)]) = bst[h];
+      *reinterpret_cast<typename E::vec2*>(&sB[buf][k][c2 ^ ((k & 1) << 4)]) = bst[h];
     }
   };
-  auto compute_store = [&](int J, int buf, double4_t(&acc)[2][2]) {
+  auto compute_store = [&](int J, int buf, typename E::acc(&acc)[2][2]) {
     const int kd = kind(J);
     const int j0 = J * 64 + jw;
     if (kd == 0) {
@@ -630,45 +656,45 @@ __device__ __forceinline__ void gj_update_body(const double* __restrict__ X, dou
       for (int kk = 0; kk < KK; ++kk) {
         const int kq = 4 * kk + lq;
         const int sw = (kq & 1) << 4;
-        double bv[2];
+        T bv[2];
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) bv[bj] = sB[buf][kq][(jw + 16 * bj + lr) ^ sw];
 #pragma unroll
         for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
           for (int bj = 0; bj < 2; ++bj)
-            acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi][kk], bv[bj], acc[bi][bj], 0, 0, 0);
+            acc[bi][bj] = E::mfma(a[bi][kk], bv[bj], acc[bi][bj]);
       }
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          double* orow = Ob + (size_t)(i0 + 16 * bi + mfma_drow(lq, r)) * ldo + j0 + lr;
+          T* orow = Ob + (size_t)(i0 + 16 * bi + E::drow(lq, r)) * ldo + j0 + lr;
 #pragma unroll
           for (int bj = 0; bj < 2; ++bj) orow[16 * bj] = acc[bi][bj][r];
         }
     } else if (kd == 1) {  // coefficient columns: Out[i][j0 + c] = AT[j0 - k0 + c][i]
       const int kb = j0 - k0;
-      double pv[2][4][2];
+      T pv[2][4][2];
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
           for (int bj = 0; bj < 2; ++bj)
-            pv[bi][r][bj] = buffer_load_f64(rA, 8 * ((kb + 16 * bj + lr) * Np + i0 + 16 * bi + mfma_drow(lq, r)));
+            pv[bi][r][bj] = load1<T>(rA, SZ * ((kb + 16 * bj + lr) * Np + i0 + 16 * bi + E::drow(lq, r)));
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          double* orow = Ob + (size_t)(i0 + 16 * bi + mfma_drow(lq, r)) * ldo + j0 + lr;
+          T* orow = Ob + (size_t)(i0 + 16 * bi + E::drow(lq, r)) * ldo + j0 + lr;
 #pragma unroll
           for (int bj = 0; bj < 2; ++bj) orow[16 * bj] = pv[bi][r][bj];
         }
     }
   };
 
-  double4_t accA[2][2], accB[2][2];
+  typename E::acc accA[2][2], accB[2][2];
   load_b(jt0, true);
   load_c(jt0, true, accA);
   store_b(0);
@@ -722,32 +748,32 @@ __device__ __forceinline__ void xcd_matrix_map(int& b, int& w) {
 
 // NBANDS = 1: 256 threads, one band per workgroup; NBANDS = 2: 512 threads, two
 // bands sharing each B tile (half the B-row traffic).
-template <int KB, int NBANDS>
+template <int KB, int NBANDS, typename T>
 __global__ __launch_bounds__(256 * NBANDS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_update(
-    const double* __restrict__ X, double* __restrict__ Out, long long ostride, int ldo, int ocol0, int Np, int k0,
-    const double* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1,
+    const T* __restrict__ X, T* __restrict__ Out, long long ostride, int ldo, int ocol0, int Np, int k0,
+    const T* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1,
     int ngroups) {
-  __shared__ double sB[2][KB][64];
+  __shared__ T sB[2][KB][64];
   int b, w;
   xcd_matrix_map(b, w);
   int jt0, jt1;
   group_tiles(T0, T1, w % ngroups, ngroups, jt0, jt1);
   if (jt0 >= jt1) return;
-  gj_update_body<KB, NBANDS>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, b, NBANDS * (w / ngroups), jt0,
+  gj_update_body<KB, NBANDS, false, T>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, b, NBANDS * (w / ngroups), jt0,
                              jt1, threadIdx.x, sB);
 }
 
 // One 64-column tile per workgroup (the look-ahead of the next pair's columns):
 // without the next tile's prefetch registers and B buffer the kernel fits four
 // waves per SIMD, so more of its latency-bound workgroups run at once.
-template <int KB>
+template <int KB, typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gj_update_one(
-    const double* __restrict__ X, double* __restrict__ Out, long long ostride, int ldo, int ocol0, int Np, int k0,
-    const double* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0) {
-  __shared__ double sB[1][KB][64];
+    const T* __restrict__ X, T* __restrict__ Out, long long ostride, int ldo, int ocol0, int Np, int k0,
+    const T* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0) {
+  __shared__ T sB[1][KB][64];
   int b, w;
   xcd_matrix_map(b, w);
-  gj_update_body<KB, 1, true>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, b, w, T0, T0 + 1, threadIdx.x, sB);
+  gj_update_body<KB, 1, true, T>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, b, w, T0, T0 + 1, threadIdx.x, sB);
 }
 
 // One launch with two jobs on disjoint data: the first `npanel` workgroups run
@@ -756,16 +782,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // 64-row bands per workgroup.  Workgroups dispatch in index order, so the
 // latency-bound panel chains take their CUs first and the update fills the
 // others -- the panel chain of pair q+1 hides behind update q.
-template <int RPT>
+template <int RPT, typename T>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_fused(
-    PairJob P, int npanel, const double* __restrict__ X, double* __restrict__ Out, int Np, int k0,
-    const double* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1,
+    PairJob<T> P, int npanel, const T* __restrict__ X, T* __restrict__ Out, int Np, int k0,
+    const T* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1,
     int ngroups) {
   if ((int)blockIdx.x < npanel) {
-    gj_pair_chain_body<RPT>(P, blockIdx.x);
+    gj_pair_chain_body<RPT, T>(P, blockIdx.x);
     return;
   }
-  __shared__ double sB[2][2 * NB][64];
+  __shared__ T sB[2][2 * NB][64];
   const int u = blockIdx.x - npanel;
   const int band_pairs = ((Np + 63) / 64 + 1) / 2;
   const int per_b = band_pairs * ngroups;
@@ -785,18 +811,20 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   group_tiles(T0, T1, rem % ngroups, ngroups, jt0, jt1);
   if (jt0 >= jt1) return;
   // a band past Np (odd band count) only stages B tiles and joins the barriers
-  gj_update_body<2 * NB, 2>(X, Out, (long long)Np * Np, Np, 0, Np, k0, AT, src2, brow, b, 2 * (rem / ngroups), jt0,
+  gj_update_body<2 * NB, 2, false, T>(X, Out, (long long)Np * Np, Np, 0, Np, k0, AT, src2, brow, b, 2 * (rem / ngroups), jt0,
                             jt1, threadIdx.x, sB);
 }
 
-__global__ void k_unpermute(const double* __restrict__ in, double* __restrict__ out, int Np,
+// out[b][r][c] = in[b][r][pinv[c]], converted to the output type.
+template <typename Tin, typename Tout>
+__global__ void k_unpermute(const Tin* __restrict__ in, Tout* __restrict__ out, int Np,
                             const int* __restrict__ pinv) {
   const int b = blockIdx.y;
   const size_t n2 = (size_t)Np * Np;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n2; idx += (size_t)gridDim.x * blockDim.x) {
     const int r = (int)(idx / Np);
     const int c = (int)(idx % Np);
-    out[b * n2 + idx] = in[b * n2 + (size_t)r * Np + pinv[(size_t)b * Np + c]];
+    out[b * n2 + idx] = (Tout)in[b * n2 + (size_t)r * Np + pinv[(size_t)b * Np + c]];
   }
 }
 
@@ -854,8 +882,9 @@ int profile_end(double* total_ms, int* count, double* total_bytes, double* total
 }
 
 namespace {
+template <typename T>
 struct PanelMaps {  // optional outputs of a panel launch
-  double* PoutT = nullptr;
+  T* PoutT = nullptr;
   long long tstride = 0;
   const int* prev_src = nullptr;
   const int* prev_piv = nullptr;
@@ -872,34 +901,35 @@ constexpr int kWalkUpdate = 2048;  // standalone update launches
 thread_local int t_walk_fused = 0;
 thread_local int t_walk_update = 0;
 
-int launch_panel(const double* X, long long bstride, int ld, int col0, int batch, int Np, int k0, double* pout,
-                 int* src, int* piv, int* perm, int* pinv, const PanelMaps& m, hipStream_t stream) {
+template <typename T>
+int launch_panel(const T* X, long long bstride, int ld, int col0, int batch, int Np, int k0, T* pout,
+                 int* src, int* piv, int* perm, int* pinv, const PanelMaps<T>& m, hipStream_t stream) {
 #define TICC_PANEL_ARGS                                                                                          \
   X, bstride, ld, col0, Np, k0, pout, src, piv, perm, pinv, m.PoutT, m.tstride, m.prev_src, m.prev_piv, m.prev_k0, \
       m.src2, m.brow
   if (Np <= 256)
-    hipLaunchKernelGGL((k_gj_panel<256, 1>), dim3(batch), dim3(256), 0, stream, TICC_PANEL_ARGS);
+    hipLaunchKernelGGL((k_gj_panel<256, 1, T>), dim3(batch), dim3(256), 0, stream, TICC_PANEL_ARGS);
   else if (Np <= 512)
-    hipLaunchKernelGGL((k_gj_panel<512, 1>), dim3(batch), dim3(512), 0, stream, TICC_PANEL_ARGS);
+    hipLaunchKernelGGL((k_gj_panel<512, 1, T>), dim3(batch), dim3(512), 0, stream, TICC_PANEL_ARGS);
   else if (Np <= 1024)
-    hipLaunchKernelGGL((k_gj_panel<512, 2>), dim3(batch), dim3(512), 0, stream, TICC_PANEL_ARGS);
+    hipLaunchKernelGGL((k_gj_panel<512, 2, T>), dim3(batch), dim3(512), 0, stream, TICC_PANEL_ARGS);
   else
-    hipLaunchKernelGGL((k_gj_panel<512, 3>), dim3(batch), dim3(512), 0, stream, TICC_PANEL_ARGS);
+    hipLaunchKernelGGL((k_gj_panel<512, 3, T>), dim3(batch), dim3(512), 0, stream, TICC_PANEL_ARGS);
 #undef TICC_PANEL_ARGS
   TICC_CHECK_LAUNCH();
   return TICC_OK;
 }
 
-template <int KB>
-int launch_update(const double* cur, double* out, long long ostride, int ldo, int ocol0, int batch, int Np, int k0,
-                  const double* AT, const int* src2, const int* brow, int T0, int T1, hipStream_t stream) {
+template <int KB, typename T>
+int launch_update(const T* cur, T* out, long long ostride, int ldo, int ocol0, int batch, int Np, int k0,
+                  const T* AT, const int* src2, const int* brow, int T0, int T1, hipStream_t stream) {
   // standalone launches run one band per workgroup (faster alone than the
   // two-band form, which pays off beside a panel in k_gj_fused)
   constexpr int NBANDS = 1;
   const int units = ((Np + 63) / 64 + NBANDS - 1) / NBANDS;  // band groups of NBANDS bands
   const int span = T1 - T0;
   if (span == 1) {
-    hipLaunchKernelGGL((k_gj_update_one<KB>), dim3(units, batch), dim3(256), 0, stream, cur, out, ostride, ldo, ocol0,
+    hipLaunchKernelGGL((k_gj_update_one<KB, T>), dim3(units, batch), dim3(256), 0, stream, cur, out, ostride, ldo, ocol0,
                        Np, k0, AT, src2, brow, T0);
     TICC_CHECK_LAUNCH();
     return TICC_OK;
@@ -909,19 +939,21 @@ int launch_update(const double* cur, double* out, long long ostride, int ldo, in
   int ngroups = (walk / NBANDS + units * batch - 1) / (units * batch);
   if (ngroups > span) ngroups = span;
   if (ngroups < 1) ngroups = 1;
-  hipLaunchKernelGGL((k_gj_update<KB, NBANDS>), dim3(units * ngroups, batch), dim3(256 * NBANDS), 0, stream, cur, out,
+  hipLaunchKernelGGL((k_gj_update<KB, NBANDS, T>), dim3(units * ngroups, batch), dim3(256 * NBANDS), 0, stream, cur, out,
                      ostride, ldo, ocol0, Np, k0, AT, src2, brow, T0, T1, ngroups);
   TICC_CHECK_LAUNCH();
   return TICC_OK;
 }
 
+template <typename T>
 struct Carve {  // scratch carve-up of gj_invert_raw
-  double *P1, *P2, *Z, *L, *AT[2];
+  T *P1, *P2, *Z, *L, *AT[2];
   int *s1, *s2, *src2[2], *p1, *p2, *brow[2];
 };
 
-Carve carve(double* pout, int* src, int* piv, int batch, int Np) {
-  Carve c;
+template <typename T>
+Carve<T> carve(T* pout, int* src, int* piv, int batch, int Np) {
+  Carve<T> c;
   const size_t ps = (size_t)batch * Np * NB;
   c.P1 = pout;
   c.P2 = pout + ps;
@@ -942,9 +974,10 @@ Carve carve(double* pout, int* src, int* piv, int batch, int Np) {
   return c;
 }
 
-PairJob pair_job(const double* Xc, long long xs, int ldx, int xcol, int Np, int k0, const Carve& c, int par,
-                 int* perm, int* pinv) {
-  PairJob P;
+template <typename T>
+PairJob<T> pair_job(const T* Xc, long long xs, int ldx, int xcol, int Np, int k0, const Carve<T>& c, int par,
+                    int* perm, int* pinv) {
+  PairJob<T> P;
   P.Xc = Xc;
   P.xs = xs;
   P.ldx = ldx;
@@ -967,34 +1000,37 @@ PairJob pair_job(const double* Xc, long long xs, int ldx, int xcol, int Np, int 
   return P;
 }
 
-int launch_pair_chain(const PairJob& P, int batch, hipStream_t stream) {
+template <typename T>
+int launch_pair_chain(const PairJob<T>& P, int batch, hipStream_t stream) {
   if (P.Np <= 512)
-    hipLaunchKernelGGL((k_gj_pair_chain<1>), dim3(batch), dim3(512), 0, stream, P);
+    hipLaunchKernelGGL((k_gj_pair_chain<1, T>), dim3(batch), dim3(512), 0, stream, P);
   else if (P.Np <= 1024)
-    hipLaunchKernelGGL((k_gj_pair_chain<2>), dim3(batch), dim3(512), 0, stream, P);
+    hipLaunchKernelGGL((k_gj_pair_chain<2, T>), dim3(batch), dim3(512), 0, stream, P);
   else
-    hipLaunchKernelGGL((k_gj_pair_chain<3>), dim3(batch), dim3(512), 0, stream, P);
+    hipLaunchKernelGGL((k_gj_pair_chain<3, T>), dim3(batch), dim3(512), 0, stream, P);
   TICC_CHECK_LAUNCH();
   return TICC_OK;
 }
 
-int single_step(double* cur, double* nxt, int batch, int Np, int k0, const Carve& c, int* perm, int* pinv,
+template <typename T>
+int single_step(T* cur, T* nxt, int batch, int Np, int k0, const Carve<T>& c, int* perm, int* pinv,
                 hipStream_t st) {
   const long long n2 = (long long)Np * Np;
-  PanelMaps m;
+  PanelMaps<T> m;
   m.PoutT = c.AT[0];
   m.tstride = 2LL * NB * Np;
   m.src2 = c.src2[0];
   m.brow = c.brow[0];
   TICC_RETURN_IF(launch_panel(cur, n2, Np, k0, batch, Np, k0, c.P1, c.s1, c.p1, perm, pinv, m, st));
-  return launch_update<NB>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[0], c.src2[0], c.brow[0], 0, (Np + 63) / 64, st);
+  return launch_update<NB, T>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[0], c.src2[0], c.brow[0], 0, (Np + 63) / 64, st);
 }
 
 // Fused launch: the panel chain P beside column tiles [T0, T1) of the rank-2NB
 // update of the pair at k0.
 // `slot` >= 0: the profile slot reserved for this launch (its events, bytes and
 // flop entries), -1: not timed.
-int launch_fused(const PairJob& P, const double* X, double* Out, int batch, int Np, int k0, const double* AT,
+template <typename T>
+int launch_fused(const PairJob<T>& P, const T* X, T* Out, int batch, int Np, int k0, const T* AT,
                  const int* src2, const int* brow, int T0, int T1, long long slot, hipStream_t stream) {
   const int band_pairs = ((Np + 63) / 64 + 1) / 2;
   // about two workgroups per CU: longer tile walks amortise each band's
@@ -1010,33 +1046,33 @@ int launch_fused(const PairJob& P, const double* X, double* Out, int batch, int 
     // 2NB (or NB) look-ahead columns and writes 2NB coefficient columns
     const double cols = (double)((T1 * 64 < Np ? T1 * 64 : Np) - T0 * 64);
     const double kp = P.single ? NB : 2 * NB;
-    g_prof.bytes[slot] = 8.0 * batch * (2.0 * Np * cols + 2.0 * NB * Np + 2.0 * kp * Np);
+    g_prof.bytes[slot] = (double)sizeof(T) * batch * (2.0 * Np * cols + 2.0 * NB * Np + 2.0 * kp * Np);
     g_prof.flop[slot] = 2.0 * batch * Np * cols * 2 * NB;
     TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot], stream));
   }
   if (Np <= 512)
-    hipLaunchKernelGGL((k_gj_fused<1>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
+    hipLaunchKernelGGL((k_gj_fused<1, T>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
                        T0, T1, ngroups);
   else if (Np <= 1024)
-    hipLaunchKernelGGL((k_gj_fused<2>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
+    hipLaunchKernelGGL((k_gj_fused<2, T>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
                        T0, T1, ngroups);
   else
-    hipLaunchKernelGGL((k_gj_fused<3>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
+    hipLaunchKernelGGL((k_gj_fused<3, T>), dim3(grid), dim3(512), 0, stream, P, batch, X, Out, Np, k0, AT, src2, brow,
                        T0, T1, ngroups);
   TICC_CHECK_LAUNCH();
   if (slot >= 0) TICC_CHECK_HIP(hipEventRecord(g_prof.events[2 * slot + 1], stream));
   return TICC_OK;
 }
-}  // namespace
 
-int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result, int* perm, int* pinv,
-                  double* pout, int* src, int* piv, hipStream_t stream) {
+template <typename T>
+int gj_invert_t(T* bufA, T* bufB, int batch, int Np, T** result, int* perm, int* pinv, T* pout, int* src, int* piv,
+                hipStream_t stream) {
   if (Np % NB != 0 || Np > MAX_NP || Np <= 0 || batch <= 0)
     return fail(TICC_ERR_VALUE, "gj_invert_raw: invalid Np=%d batch=%d", Np, batch);
-  const Carve c = carve(pout, src, piv, batch, Np);
+  const Carve<T> c = carve<T>(pout, src, piv, batch, Np);
   const long long n2 = (long long)Np * Np;
   const int tiles = (Np + 63) / 64;
-  double* buf[2] = {bufA, bufB};
+  T* buf[2] = {bufA, bufB};
   const int steps = Np / NB;
   const int pairs = steps / 2;
   // Profile slots of this inversion's fused launches (one per pair but the last,
@@ -1053,28 +1089,28 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
   }
   auto next_slot = [&]() { return slot < 0 ? -1LL : slot++; };
   const long long ln = 2LL * NB * Np;  // look-ahead buffer: Np x 2NB per matrix
-  if (pairs > 0) TICC_RETURN_IF(launch_pair_chain(pair_job(buf[0], n2, Np, 0, Np, 0, c, 0, perm, pinv), batch, stream));
+  if (pairs > 0) TICC_RETURN_IF(launch_pair_chain(pair_job<T>(buf[0], n2, Np, 0, Np, 0, c, 0, perm, pinv), batch, stream));
   for (int q = 0; q < pairs; ++q) {
     const int k0 = 2 * q * NB;
     const int par = q & 1;
-    double* cur = buf[q & 1];
-    double* nxt = buf[(q + 1) & 1];
+    T* cur = buf[q & 1];
+    T* nxt = buf[(q + 1) & 1];
     if (q + 1 == pairs) {
       if (!(steps & 1)) {
-        TICC_RETURN_IF(launch_update<2 * NB>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[par], c.src2[par],
+        TICC_RETURN_IF(launch_update<2 * NB, T>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[par], c.src2[par],
                                              c.brow[par], 0, tiles, stream));
         break;
       }
       // trailing single step: its panel (on a look-ahead of its columns) runs
       // inside the last pair's update, then the rank-NB update
       const int ks = k0 + 2 * NB;
-      TICC_RETURN_IF(launch_update<2 * NB>(cur, c.L, ln, 2 * NB, ks, batch, Np, k0, c.AT[par], c.src2[par],
+      TICC_RETURN_IF(launch_update<2 * NB, T>(cur, c.L, ln, 2 * NB, ks, batch, Np, k0, c.AT[par], c.src2[par],
                                            c.brow[par], ks / 64, ks / 64 + 1, stream));
-      PairJob P = pair_job(c.L, ln, 2 * NB, 0, Np, ks, c, par ^ 1, perm, pinv);
+      PairJob<T> P = pair_job<T>(c.L, ln, 2 * NB, 0, Np, ks, c, par ^ 1, perm, pinv);
       P.single = 1;
       TICC_RETURN_IF(
           launch_fused(P, cur, nxt, batch, Np, k0, c.AT[par], c.src2[par], c.brow[par], 0, tiles, next_slot(), stream));
-      TICC_RETURN_IF(launch_update<NB>(nxt, cur, n2, Np, 0, batch, Np, ks, c.AT[par ^ 1], c.src2[par ^ 1],
+      TICC_RETURN_IF(launch_update<NB, T>(nxt, cur, n2, Np, 0, batch, Np, ks, c.AT[par ^ 1], c.src2[par ^ 1],
                                        c.brow[par ^ 1], 0, tiles, stream));
       *result = cur;
       return TICC_OK;
@@ -1082,18 +1118,30 @@ int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result
     // The panel chain of pair q+1 (columns kn, kn + NB) runs inside update q:
     // its columns after pair q come from a one-tile look-ahead of update q.
     const int kn = k0 + 2 * NB;
-    TICC_RETURN_IF(launch_update<2 * NB>(cur, c.L, ln, 2 * NB, kn, batch, Np, k0, c.AT[par], c.src2[par],
+    TICC_RETURN_IF(launch_update<2 * NB, T>(cur, c.L, ln, 2 * NB, kn, batch, Np, k0, c.AT[par], c.src2[par],
                                          c.brow[par], kn / 64, kn / 64 + 1, stream));
-    TICC_RETURN_IF(launch_fused(pair_job(c.L, ln, 2 * NB, 0, Np, kn, c, par ^ 1, perm, pinv), cur, nxt, batch, Np, k0,
+    TICC_RETURN_IF(launch_fused(pair_job<T>(c.L, ln, 2 * NB, 0, Np, kn, c, par ^ 1, perm, pinv), cur, nxt, batch, Np, k0,
                                 c.AT[par], c.src2[par], c.brow[par], 0, tiles, next_slot(), stream));
   }
-  double* cur = buf[pairs & 1];
+  T* cur = buf[pairs & 1];
   if (steps & 1) {  // trailing single step: rank-NB update
     TICC_RETURN_IF(single_step(cur, buf[(pairs + 1) & 1], batch, Np, (steps - 1) * NB, c, perm, pinv, stream));
     cur = buf[(pairs + 1) & 1];
   }
   *result = cur;
   return TICC_OK;
+}
+
+}  // namespace
+
+int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result, int* perm, int* pinv, double* pout,
+                  int* src, int* piv, hipStream_t stream) {
+  return gj_invert_t<double>(bufA, bufB, batch, Np, result, perm, pinv, pout, src, piv, stream);
+}
+
+int gj_invert_raw_f32(float* bufA, float* bufB, int batch, int Np, float** result, int* perm, int* pinv, float* pout,
+                      int* src, int* piv, hipStream_t stream) {
+  return gj_invert_t<float>(bufA, bufB, batch, Np, result, perm, pinv, pout, src, piv, stream);
 }
 
 int gj_set_walk(int fused_target, int update_target) {
@@ -1104,13 +1152,28 @@ int gj_set_walk(int fused_target, int update_target) {
   return TICC_OK;
 }
 
-int gj_unpermute(const double* in, double* out, int batch, int Np, const int* pinv, hipStream_t stream) {
+namespace {
+template <typename Tin, typename Tout>
+int unpermute_t(const Tin* in, Tout* out, int batch, int Np, const int* pinv, hipStream_t stream) {
   const long long n2 = (long long)Np * Np;
   int blocks = (int)((n2 + 255) / 256);
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_unpermute, dim3(blocks, batch), dim3(256), 0, stream, in, out, Np, pinv);
+  hipLaunchKernelGGL((k_unpermute<Tin, Tout>), dim3(blocks, batch), dim3(256), 0, stream, in, out, Np, pinv);
   TICC_CHECK_LAUNCH();
   return TICC_OK;
+}
+}  // namespace
+
+int gj_unpermute(const double* in, double* out, int batch, int Np, const int* pinv, hipStream_t stream) {
+  return unpermute_t<double, double>(in, out, batch, Np, pinv, stream);
+}
+
+int gj_unpermute_f32(const float* in, double* out, int batch, int Np, const int* pinv, hipStream_t stream) {
+  return unpermute_t<float, double>(in, out, batch, Np, pinv, stream);
+}
+
+int gj_unpermute_f32f32(const float* in, float* out, int batch, int Np, const int* pinv, hipStream_t stream) {
+  return unpermute_t<float, float>(in, out, batch, Np, pinv, stream);
 }
 
 }  // namespace ticc

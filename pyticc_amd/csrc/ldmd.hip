@@ -298,6 +298,18 @@ __global__ void k_init_capture(const double* __restrict__ W0, const double* __re
   }
 }
 
+// FP32 copy of an FP64 batch (the mixed sweep's input to the FP32 inversion).
+__global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = (float)in[i];
+}
+
+// The identity column map (a plain inverse needs no unpermute).
+__global__ void k_iota(int* __restrict__ p, int Np, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = (int)(i % Np);
+}
+
 int grid1(long long n) {
   long long b = (n + 255) / 256;
   if (b > 2048) b = 2048;
@@ -305,6 +317,12 @@ int grid1(long long n) {
 }
 
 }  // namespace
+
+int to_f32(const double* in, float* out, long long n, hipStream_t stream) {
+  hipLaunchKernelGGL(k_to_f32, dim3(grid1(n)), dim3(256), 0, stream, in, out, n);
+  TICC_CHECK_LAUNCH();
+  return TICC_OK;
+}
 
 int ldmd_init(int nE, int N, const double* W0, const double* E_host, double mu, double* Y, hipStream_t stream) {
   void* ws = nullptr;
@@ -332,7 +350,7 @@ int ldmd_init_capture(int nE, int N, const double* W0, const double* E_host, dou
 
 int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const double* hs_host, const double* Ws0,
                    const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream, int half,
-                   int n_block, long long gstride) {
+                   int n_block, long long gstride, int refine) {
   const int Np = round_up(N, GJ_NB);
   if (Np > MAX_NP) return fail(TICC_ERR_UNSUPPORTED, "n_channel=%d exceeds the supported maximum %d", N, MAX_NP);
   if (n_block < 1 || nE % n_block) return fail(TICC_ERR_VALUE, "the batch must hold the same energies for every block");
@@ -364,11 +382,27 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   const size_t oE = take(sizeof(double) * nE);
   const size_t oH = take(sizeof(double) * nS);
   const size_t oFlag = take(sizeof(int));
+  // refine >= 0: the mixed-precision sweep -- each sector matrix is inverted
+  // in FP32 and refined by `refine` Newton steps X <- X + X (I - A X), the
+  // residual I - A X on the FP64 MFMA from the FP64 A, the correction X (I - A X)
+  // on the FP32 MFMA; everything else (forms, reference values, Q_mid, Y) stays
+  // FP64.  refine < 0: the FP64 sweep.
+  const bool mixed = refine >= 0;
   struct Scratch {
     size_t A0, A1, P, src, piv, perm, pinv, y1, y2;
+    size_t F0, F1, X, T32, X32[2], ipinv;  // mixed sweep only
   };
   auto scratch = [&](int batch, bool yv) {
     Scratch sc{};
+    if (yv && mixed) {
+      sc.F0 = take(sizeof(float) * batch * n2);
+      sc.F1 = take(sizeof(float) * batch * n2);
+      sc.X = take(sizeof(double) * batch * n2);
+      sc.T32 = take(sizeof(float) * batch * n2);
+      sc.X32[0] = take(sizeof(float) * batch * n2);
+      sc.X32[1] = take(sizeof(float) * batch * n2);
+      sc.ipinv = take(sizeof(int) * batch * Np);
+    }
     sc.A0 = yv ? take(sizeof(double) * batch * n2) : 0;
     sc.A1 = yv ? take(sizeof(double) * batch * n2) : 0;
     sc.P = take(sizeof(double) * GJ_POUT_PANELS * batch * Np * GJ_NB);
@@ -389,6 +423,7 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   char* w = static_cast<char*>(base);
   auto D = [&](size_t o) { return reinterpret_cast<double*>(w + o); };
   auto I = [&](size_t o) { return reinterpret_cast<int*>(w + o); };
+  auto F = [&](size_t o) { return reinterpret_cast<float*>(w + o); };
   double* Yp = D(oY);
   double* Qm = D(oQ);
   double* M0 = D(oM);
@@ -408,6 +443,12 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   TICC_CHECK_HIP(hipMemcpyAsync(dH, hs_host, sizeof(double) * nS, hipMemcpyHostToDevice, stream));
   TICC_CHECK_HIP(hipMemsetAsync(flag, 0, sizeof(int), stream));
   TICC_RETURN_IF(pad_identity(Y, (long long)N * N, N, Yp, Np, nE, stream));
+  if (mixed)
+    for (int g = 0; g < G; ++g) {
+      hipLaunchKernelGGL(k_iota, dim3(grid1((long long)per * Np)), dim3(256), 0, stream, I(sg[g].ipinv), Np,
+                         (long long)per * Np);
+      TICC_CHECK_LAUNCH();
+    }
 
   const double two_mu = 2.0 * mu;
   const int gelem = grid1((long long)n2);
@@ -450,6 +491,32 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
                            Np, dE + e0, two_mu, ne, ys(sc.y1, s), ys(sc.y2, s), e0, per_block, gstride);
         return hipGetLastError();
       };
+      // Inverse of the sector matrices in Ain (Aalt: the FP64 inversion's ping-pong
+      // partner): *X receives the buffer holding it and *pv its column map.
+      auto invert = [&](double* Ain, double* Aalt, double** X, const int** pv) -> int {
+        if (!mixed) {
+          TICC_RETURN_IF(gj_invert_raw(Ain, Aalt, ne, Np, X, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src), I(sc.piv), st));
+          *pv = I(sc.pinv);
+          return TICC_OK;
+        }
+        const long long nn = (long long)ne * n2;
+        hipLaunchKernelGGL(k_to_f32, dim3(grid1(nn)), dim3(256), 0, st, Ain, F(sc.F0), nn);
+        TICC_CHECK_LAUNCH();
+        float* r32 = nullptr;
+        TICC_RETURN_IF(gj_invert_raw_f32(F(sc.F0), F(sc.F1), ne, Np, &r32, I(sc.perm), I(sc.pinv),
+                                         reinterpret_cast<float*>(D(sc.P)), I(sc.src), I(sc.piv), st));
+        TICC_RETURN_IF(gj_unpermute_f32(r32, D(sc.X), ne, Np, I(sc.pinv), st));
+        if (refine > 0) TICC_RETURN_IF(gj_unpermute_f32f32(r32, F(sc.X32[0]), ne, Np, I(sc.pinv), st));
+        for (int it = 0; it < refine; ++it) {
+          // T = I - A X (FP64 operands, FP32 result), X <- X + X T (FP32 operands)
+          TICC_RETURN_IF(refine_gemm_f64(Np, ne, Ain, D(sc.X), -1.0, nullptr, 0.0, 1.0, nullptr, F(sc.T32), st));
+          TICC_RETURN_IF(refine_gemm_f32(Np, ne, F(sc.X32[it & 1]), F(sc.T32), 1.0, D(sc.X), 1.0, 0.0, D(sc.X),
+                                         it + 1 < refine ? F(sc.X32[(it + 1) & 1]) : nullptr, st));
+        }
+        *X = D(sc.X);
+        *pv = I(sc.ipinv);
+        return TICC_OK;
+      };
       TICC_CHECK_HIP(hipStreamWaitEvent(st, ev_qm[qb], 0));
       for (int q = 0; q < nc; ++q) {
         const int s = s0 + q;
@@ -467,24 +534,28 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
           a_alt[g] = A1;
         }
         double* r1 = nullptr;
-        TICC_RETURN_IF(
-            gj_invert_raw(a_in[g], a_alt[g], ne, Np, &r1, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src), I(sc.piv), st));
-        double* other = (r1 == A0) ? A1 : A0;
-        hipLaunchKernelGGL(k_form2, dim3(gelem, ne), dim3(256), 0, st, r1, I(sc.pinv), Qbuf + q * n2, y1, y2, N, Np,
-                           other, e0, per_block, qstride);
+        const int* pv1 = nullptr;
+        TICC_RETURN_IF(invert(a_in[g], a_alt[g], &r1, &pv1));
+        // A2 goes where neither A1's inverse nor (mixed) A1 itself is needed again
+        double* other = mixed ? a_alt[g] : ((r1 == A0) ? A1 : A0);
+        hipLaunchKernelGGL(k_form2, dim3(gelem, ne), dim3(256), 0, st, r1, pv1, Qbuf + q * n2, y1, y2, N, Np, other,
+                           e0, per_block, qstride);
         TICC_CHECK_LAUNCH();
         double* r2 = nullptr;
-        TICC_RETURN_IF(gj_invert_raw(other, r1, ne, Np, &r2, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src), I(sc.piv), st));
+        const int* pv2 = nullptr;
+        TICC_RETURN_IF(invert(other, mixed ? a_in[g] : r1, &r2, &pv2));
         NextSector nx{nullptr, nullptr, 0.0, y1, nullptr};
-        if (s + 1 < nS) {  // form the next sector's A1 in the buffer X2 does not occupy
+        if (s + 1 < nS) {  // form the next sector's A1 in a buffer X2 does not occupy
           TICC_CHECK_HIP(refvals(s + 1));
-          nx = NextSector{Ws0 + (s + 1) * wstride, Wm0 + (s + 1) * wstride, hs_host[s + 1], ys(sc.y1, s + 1),
-                          (r2 == A0) ? A1 : A0};
-          a_in[g] = nx.A;
-          a_alt[g] = r2;
+          double* nA = mixed ? a_in[g] : ((r2 == A0) ? A1 : A0);
+          nx = NextSector{Ws0 + (s + 1) * wstride, Wm0 + (s + 1) * wstride, hs_host[s + 1], ys(sc.y1, s + 1), nA};
+          if (!mixed) {
+            a_in[g] = nA;
+            a_alt[g] = r2;
+          }
         }
-        hipLaunchKernelGGL(k_form3, dim3(tn * (tn + 1) / 2, ne), dim3(256), 0, st, r2, I(sc.pinv), We, Wm, h, y1, y2, N,
-                           Np, half, Yg, e0, per_block, gstride, nx);
+        hipLaunchKernelGGL(k_form3, dim3(tn * (tn + 1) / 2, ne), dim3(256), 0, st, r2, pv2, We, Wm, h, y1, y2, N, Np,
+                           half, Yg, e0, per_block, gstride, nx);
         TICC_CHECK_LAUNCH();
       }
       TICC_CHECK_HIP(hipEventRecord(ev_done[2 * g + qb], st));

@@ -52,9 +52,9 @@ inline int round_up(int n, int q) { return (n + q - 1) / q * q; }
 
 #define TICC_CHECK_LAUNCH() TICC_CHECK_HIP(hipGetLastError())
 
-#define TICC_RETURN_IF(status)          \
+#define TICC_RETURN_IF(...)             \
   do {                                  \
-    int s_ = (status);                  \
+    int s_ = (__VA_ARGS__);             \
     if (s_ != TICC_OK) return s_;       \
   } while (0)
 
@@ -65,6 +65,41 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // D[mfma_drow(l/16, r)][l%16].
 __host__ __device__ constexpr int mfma_drow(int lq, int r) { return lq + 4 * r; }
 
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+namespace ticc {
+// Element-type helpers of the kernels that run in FP64 or FP32 (the
+// Gauss-Jordan inversion, the refinement GEMMs): the 16x16x4 MFMA of the type,
+// whose output row of accumulator register r in lane group lq differs between
+// the two, FMA and two-element vectors.
+template <typename T>
+struct Elem;
+template <>
+struct Elem<double> {
+  typedef double4_t acc;
+  typedef double2 vec2;
+  __device__ static __forceinline__ acc mfma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static __forceinline__ constexpr int drow(int lq, int r) { return mfma_drow(lq, r); }
+  __device__ static __forceinline__ double fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+  __device__ static __forceinline__ vec2 pair(double x, double y) { return make_double2(x, y); }
+};
+template <>
+struct Elem<float> {
+  typedef float4_t acc;
+  typedef float2 vec2;
+  __device__ static __forceinline__ acc mfma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // v_mfma_f32_16x16x4f32: lane l, register r holds D[4 (l / 16) + r][l % 16]
+  __device__ static __forceinline__ constexpr int drow(int lq, int r) { return 4 * lq + r; }
+  __device__ static __forceinline__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+  __device__ static __forceinline__ vec2 pair(float x, float y) { return make_float2(x, y); }
+};
+
+}  // namespace ticc
+
 // Internal entry points shared between translation units.
 namespace ticc {
 // Blocked GJ inversion of `batch` padded Np x Np matrices in place in bufA
@@ -73,12 +108,27 @@ namespace ticc {
 // the column of *result that holds column c of the inverse.
 int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result, int* perm, int* pinv,
                   double* pout, int* src, int* piv, hipStream_t stream);
+// The same inversion in FP32 (FP32 panels, FP32 MFMA updates, same pivot rule).
+int gj_invert_raw_f32(float* bufA, float* bufB, int batch, int Np, float** result, int* perm, int* pinv, float* pout,
+                      int* src, int* piv, hipStream_t stream);
 // Gather the column permutation into a plain inverse: out[b][r][c] = in[b][r][pinv[c]].
 int gj_unpermute(const double* in, double* out, int batch, int Np, const int* pinv, hipStream_t stream);
+// The same gather from an FP32 inverse into FP64.
+int gj_unpermute_f32(const float* in, double* out, int batch, int Np, const int* pinv, hipStream_t stream);
+int gj_unpermute_f32f32(const float* in, float* out, int batch, int Np, const int* pinv, hipStream_t stream);
 // Batched C = alpha op(A) op(B) + beta C for padded matrices (m, n, k multiples of 32).
 int dgemm_batched(bool transA, bool transB, int m, int n, int k, double alpha, const double* A, int lda,
                   long long strideA, const double* B, int ldb, long long strideB, double beta, double* C, int ldc,
                   long long strideC, int batch, hipStream_t stream);
+// Batched square GEMMs of the Newton refinement (Np a multiple of 32):
+// out = beta Cin + alpha A B + diag I, out32 = (float) out; either output may be
+// null.  FP64 operands on the FP64 MFMA, or FP32 operands on the FP32 MFMA.
+int refine_gemm_f64(int Np, int batch, const double* A, const double* B, double alpha, const double* Cin, double beta,
+                    double diag, double* out, float* out32, hipStream_t stream);
+int refine_gemm_f32(int Np, int batch, const float* A, const float* B, double alpha, const double* Cin, double beta,
+                    double diag, double* out, float* out32, hipStream_t stream);
+// FP32 copy of n doubles.
+int to_f32(const double* in, float* out, long long n, hipStream_t stream);
 // Identity padding: dst (batch, Np, Np) <- src (batch, N, N) with ld N, identity in the padding.
 int pad_identity(const double* src, long long src_stride, int N, double* dst, int Np, int batch, hipStream_t stream);
 int unpad(const double* src, int Np, double* dst, long long dst_stride, int N, int batch, hipStream_t stream);

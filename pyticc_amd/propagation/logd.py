@@ -112,18 +112,22 @@ def initial_logD_energies(W0_base: torch.Tensor, energies, reduced_mass: float, 
     return Y
 
 
-def propagate_points(Y: torch.Tensor, energies, reduced_mass: float, half_steps, W_points: torch.Tensor, device: torch.device) -> torch.Tensor:
-    """In-place sweep of Y over the sectors spanned by W_points (2 n_S + 1, N, N)."""
+def propagate_points(Y: torch.Tensor, energies, reduced_mass: float, half_steps, W_points: torch.Tensor, device: torch.device,
+                     refine_steps: int | None = None) -> torch.Tensor:
+    """In-place sweep of Y over the sectors spanned by W_points (2 n_S + 1, N, N);
+    refine_steps set: the mixed-precision sweep with that many Newton steps."""
     E = np.ascontiguousarray(energies, dtype=np.float64)
     hs = np.ascontiguousarray(half_steps, dtype=np.float64)
     N = Y.shape[-1]
     n2 = N * N
     base = _device.ptr(W_points)
-    _backend.call(
-        "ticc_ldmd_propagate", E.size, N, E.ctypes.data_as(_backend._dp), float(reduced_mass), hs.size,
-        hs.ctypes.data_as(_backend._dp), base, base + 8 * n2, base + 16 * n2, 2 * n2, _device.ptr(Y),
-        _device.stream_handle(device),
-    )
+    args = (E.size, N, E.ctypes.data_as(_backend._dp), float(reduced_mass), hs.size, hs.ctypes.data_as(_backend._dp),
+            base, base + 8 * n2, base + 16 * n2, 2 * n2)
+    if refine_steps is None:
+        _backend.call("ticc_ldmd_propagate", *args, _device.ptr(Y), _device.stream_handle(device))
+    else:
+        _backend.call("ticc_ldmd_propagate_mixed", *args, int(refine_steps), _device.ptr(Y),
+                      _device.stream_handle(device))
     return Y
 
 

@@ -101,8 +101,11 @@ def propagate(hamiltonian, Etot, radial_sectors, config) -> torch.Tensor:
             if Y is None:
                 init = initial_logD_capture_energies if capture else initial_logD_energies
                 Y = init(Wpts[0], energies, mu, dev)
-            sweep = propagate_points_capture if capture else propagate_points
-            sweep(Y, energies, mu, hs_all[w0 : w0 + len(window)], Wpts, dev)
+            if capture:
+                propagate_points_capture(Y, energies, mu, hs_all[w0 : w0 + len(window)], Wpts, dev)
+            else:
+                propagate_points(Y, energies, mu, hs_all[w0 : w0 + len(window)], Wpts, dev,
+                                 config.refine_steps if config.precision == "mixed" else None)
             done += len(window)
             if config.print_verbose and (done == len(sectors) or done >= next_progress):
                 logger.info(f"Propagation: {done}/{len(sectors)} sectors, R={pts[-1]:.6f} bohr, wall={perf_counter() - started:.3f} s")
@@ -119,6 +122,8 @@ def propagate_blocks(hamiltonians, Etot, radial_sectors, config) -> list:
     energies = get_Etot(Etot)
     if config.mode != "inelastic":
         raise fail(ValueError, "Batched block propagation is for the inelastic mode")
+    if config.precision != "fp64":
+        raise fail(NotImplementedError, "precision='mixed' is built for exact-CC blocks swept one at a time")
     sectors = tuple(radial_sectors)
     if not sectors:
         raise fail(RuntimeError, "Propagation produced no radial windows")

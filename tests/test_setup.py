@@ -92,9 +92,13 @@ def test_sectors_and_windows():
 
 def test_propagation_config_validation():
     assert ticc.Propagation(device=" GPU:0 ").device == "gpu:0"
-    for bad in ({"mode": "x"}, {"memory_mb": 0.0}, {"device": "tpu"}, {"print_verbose": 1}):
+    for bad in ({"mode": "x"}, {"memory_mb": 0.0}, {"device": "tpu"}, {"print_verbose": 1}, {"precision": "fp32"},
+                {"refine_steps": 4}, {"refine_steps": -1}, {"refine_steps": 1.0}):
         with pytest.raises(ValueError):
             ticc.Propagation(**bad)
+    assert ticc.Propagation(precision="mixed").refine_steps == 2
+    with pytest.raises(NotImplementedError):
+        ticc.Propagation(mode="capture", precision="mixed")
 
 
 def test_system_validation_and_type_errors():
