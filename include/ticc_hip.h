@@ -149,12 +149,24 @@ int ticc_ldmd_propagate_blocks(int n_block, int n_energy, int n_channel, const d
  * updates, the same pivot rule) followed by refine_steps (0..3) Newton steps
  * X <- X + X (I - A X): the residual I - A X from the FP64 A on the FP64 MFMA,
  * the correction on the FP32 MFMA.  Reference values, corrections, Q_mid, the
- * form steps and Y stay FP64.  Two steps keep the S matrix within 1e-6 of the
- * FP64 sweep on the benchmark systems.
+ * form steps and Y stay FP64.  A matrix whose first residual max|I - A X|
+ * exceeds 1e-4 (TICC_MIXED_FALLBACK_TAU) is inverted in FP64 instead, so
+ * ill-conditioned sector matrices near log-derivative poles keep FP64 accuracy.
+ * Two steps keep the S matrix within 1e-6 of the FP64 sweep on the benchmark
+ * systems.
  */
 int ticc_ldmd_propagate_mixed(int n_energy, int n_channel, const double* energies, double reduced_mass, int n_sector,
                               const double* half_steps, const double* W_start, const double* W_mid, const double* W_end,
                               long long w_sector_stride, int refine_steps, double* Y, void* stream);
+
+/*
+ * Counters of the calling thread's mixed-precision sweeps since the last reset:
+ * the sector-matrix inversions done, and how many of them were redone in FP64
+ * because the FP32 inverse's residual max|I - A X| exceeded 1e-4 (near a
+ * log-derivative pole the sector matrix is too ill-conditioned for FP32
+ * refinement to reach FP64 accuracy).  reset != 0 zeroes them after reading.
+ */
+int ticc_mixed_stats(long long* inversions, long long* fallbacks, int reset);
 
 /*
  * Capture boundary condition (Propagation(mode="capture")): complex symmetric

@@ -54,6 +54,7 @@ _SIGNATURES = {
     "ticc_bessel_eval": ([_c_int, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "ticc_invert_batched": ([_c_int, _c_int, _vp, _vp], _c_int),
     "ticc_invert_batched_mixed": ([_c_int, _c_int, _c_int, _vp, _vp], _c_int),
+    "ticc_mixed_stats": ([_vp, _vp, _c_int], _c_int),
     "ticc_ldmd_propagate_mixed": (
         [_c_int, _c_int, _dp, _c_d, _c_int, _dp, _vp, _vp, _vp, _c_ll, _c_int, _vp, _vp],
         _c_int,

@@ -31,6 +31,7 @@ int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, con
              const double* Eint_host, const double* L_host, double* S_re, double* S_im, int* n_open,
              hipStream_t stream);
 int profile_begin(int every, int max_pairs);
+int mixed_stats(long long* inversions, long long* fallbacks, int reset);
 int gj_set_walk(int fused_target, int update_target);
 int profile_end(double* total_ms, int* count, double* total_bytes, double* total_flop);
 int bessel_eval(int n, const double* ell, const double* x, double* out, const double* nu, double* kout,
@@ -262,6 +263,11 @@ int ticc_invert_batched(int n_batch, int n_channel, double* A, void* stream) {
   return ticc::invert_with_scratch(n_batch, n_channel, A, base, as_stream(stream));
 }
 
+int ticc_mixed_stats(long long* inversions, long long* fallbacks, int reset) {
+  if (!inversions || !fallbacks) return fail(TICC_ERR_VALUE, "mixed_stats: null pointer argument");
+  return ticc::mixed_stats(inversions, fallbacks, reset);
+}
+
 int ticc_invert_batched_mixed(int n_batch, int n_channel, int refine_steps, double* A, void* stream) {
   TICC_RETURN_IF(check_channels(n_channel));
   if (n_batch < 1 || !A) return fail(TICC_ERR_VALUE, "invert_batched_mixed: invalid arguments");
@@ -298,7 +304,8 @@ int ticc_invert_batched_mixed(int n_batch, int n_channel, int refine_steps, doub
   if (refine_steps > 0) TICC_RETURN_IF(ticc::gj_unpermute_f32f32(r32, F(oX0), n_batch, Np, I(oI), st));
   float* X32[2] = {F(oX0), F(oX1)};
   for (int it = 0; it < refine_steps; ++it) {
-    TICC_RETURN_IF(ticc::refine_gemm_f64(Np, n_batch, D(oA), D(oX), -1.0, nullptr, 0.0, 1.0, nullptr, F(oT), st));
+    TICC_RETURN_IF(
+        ticc::refine_gemm_f64(Np, n_batch, D(oA), D(oX), -1.0, nullptr, 0.0, 1.0, nullptr, F(oT), nullptr, st));
     TICC_RETURN_IF(ticc::refine_gemm_f32(Np, n_batch, X32[it & 1], F(oT), 1.0, D(oX), 1.0, 0.0, D(oX),
                                          it + 1 < refine_steps ? X32[(it + 1) & 1] : nullptr, st));
   }

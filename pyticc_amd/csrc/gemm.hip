@@ -70,7 +70,9 @@ __global__ __launch_bounds__(256) void k_dgemm(int m, int n, int k, double alpha
 // Batched square GEMM of the Newton refinement of an approximate inverse
 // (Np x Np row-major, Np a multiple of 32):
 //   out[b] = beta Cin[b] + alpha A[b] B[b] + diag I,   out32[b] = (float) out[b],
-// either output optional.  TI is the operand type: FP64 operands on the FP64
+// either output optional; with rho set, rho[b] becomes at least the largest
+// |out[b]| element (as the bits of a non-negative float, which order as unsigned
+// integers).  TI is the operand type: FP64 operands on the FP64
 // MFMA, FP32 operands on the FP32 MFMA with FP32 accumulation.  A workgroup of
 // four waves owns a 64 x 64 tile of out (32 x 32 per wave) and walks K in slabs
 // of 32 staged through LDS, double-buffered: the next slab's global loads are
@@ -80,7 +82,8 @@ __global__ __launch_bounds__(256) void k_dgemm(int m, int n, int k, double alpha
 template <typename TI>
 __global__ __launch_bounds__(256) void k_gemm_sq(int Np, int tiles, const TI* __restrict__ A, const TI* __restrict__ B,
                                                  double alpha, const double* __restrict__ Cin, double beta, double diag,
-                                                 double* __restrict__ out, float* __restrict__ out32) {
+                                                 double* __restrict__ out, float* __restrict__ out32,
+                                                 unsigned* __restrict__ rho) {
   typedef Elem<TI> E;
   constexpr int KS = 32;
   __shared__ TI As[2][64][KS + 1];
@@ -152,6 +155,7 @@ __global__ __launch_bounds__(256) void k_gemm_sq(int Np, int tiles, const TI* __
     if (ks + 1 < nk) stage(buf ^ 1);
     __syncthreads();
   }
+  float vmax = 0.0f;
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -168,18 +172,24 @@ __global__ __launch_bounds__(256) void k_gemm_sq(int Np, int tiles, const TI* __
         if (i == j) v += diag;
         if (out) out[o] = v;
         if (out32) out32[o] = (float)v;
+        vmax = fmaxf(vmax, (float)fabs(v));
       }
     }
+  if (rho) {  // one atomic per wave
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+    if (lane == 0) atomicMax(rho + b, __float_as_uint(vmax));
+  }
 }
 
 template <typename TI>
 int gemm_sq(int Np, int batch, const TI* A, const TI* B, double alpha, const double* Cin, double beta, double diag,
-            double* out, float* out32, hipStream_t stream) {
+            double* out, float* out32, unsigned* rho, hipStream_t stream) {
   if (Np <= 0 || Np % 32 || batch <= 0 || (beta != 0.0 && !Cin) || (!out && !out32))
     return fail(TICC_ERR_VALUE, "gemm_sq: unsupported arguments Np=%d batch=%d", Np, batch);
   const int tiles = (Np + 63) / 64;
   hipLaunchKernelGGL((k_gemm_sq<TI>), dim3(tiles * tiles, batch), dim3(256), 0, stream, Np, tiles, A, B, alpha, Cin,
-                     beta, diag, out, out32);
+                     beta, diag, out, out32, rho);
   TICC_CHECK_LAUNCH();
   return TICC_OK;
 }
@@ -187,13 +197,13 @@ int gemm_sq(int Np, int batch, const TI* A, const TI* B, double alpha, const dou
 }  // namespace
 
 int refine_gemm_f64(int Np, int batch, const double* A, const double* B, double alpha, const double* Cin, double beta,
-                    double diag, double* out, float* out32, hipStream_t stream) {
-  return gemm_sq<double>(Np, batch, A, B, alpha, Cin, beta, diag, out, out32, stream);
+                    double diag, double* out, float* out32, unsigned* rho, hipStream_t stream) {
+  return gemm_sq<double>(Np, batch, A, B, alpha, Cin, beta, diag, out, out32, rho, stream);
 }
 
 int refine_gemm_f32(int Np, int batch, const float* A, const float* B, double alpha, const double* Cin, double beta,
                     double diag, double* out, float* out32, hipStream_t stream) {
-  return gemm_sq<float>(Np, batch, A, B, alpha, Cin, beta, diag, out, out32, stream);
+  return gemm_sq<float>(Np, batch, A, B, alpha, Cin, beta, diag, out, out32, nullptr, stream);
 }
 
 int dgemm_batched(bool transA, bool transB, int m, int n, int k, double alpha, const double* A, int lda,

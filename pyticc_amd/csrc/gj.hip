@@ -371,7 +371,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS
     const T* __restrict__ X, long long bstride, int ld, int col0, int Np, int k0, T* __restrict__ Pout,
     int* __restrict__ src, int* __restrict__ piv, int* __restrict__ perm, int* __restrict__ pinv,
     T* __restrict__ PoutT, long long tstride, const int* __restrict__ prev_src, const int* __restrict__ prev_piv,
-    int prev_k0, int* __restrict__ src2, int* __restrict__ brow) {
+    int prev_k0, int* __restrict__ src2, int* __restrict__ brow, const int* __restrict__ mask) {
+  if (mask && !mask[blockIdx.x]) return;
   gj_panel_body<THREADS, RPT, T>(X, bstride, ld, col0, Np, k0, Pout, src, piv, perm, pinv, PoutT, tstride, prev_src,
                               prev_piv, prev_k0, src2, brow, blockIdx.x);
 }
@@ -522,6 +523,7 @@ struct PairJob {
   int single;  // 1: only the panel of the trailing single step (with its update maps)
   T *P1, *P2, *Z, *AT;
   int *s1, *s2, *p1, *p2, *src2, *brow, *perm, *pinv;
+  const int* mask;  // null, or 0 for the matrices this inversion skips
 };
 
 template <int RPT, typename T>
@@ -551,6 +553,7 @@ __device__ __forceinline__ void gj_pair_chain_body(const PairJob<T>& P, int b) {
 
 template <int RPT, typename T>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_pair_chain(PairJob<T> P) {
+  if (P.mask && !P.mask[blockIdx.x]) return;
   gj_pair_chain_body<RPT, T>(P, blockIdx.x);
 }
 
@@ -752,10 +755,11 @@ template <int KB, int NBANDS, typename T>
 __global__ __launch_bounds__(256 * NBANDS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gj_update(
     const T* __restrict__ X, T* __restrict__ Out, long long ostride, int ldo, int ocol0, int Np, int k0,
     const T* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1,
-    int ngroups) {
+    int ngroups, const int* __restrict__ mask) {
   __shared__ T sB[2][KB][64];
   int b, w;
   xcd_matrix_map(b, w);
+  if (mask && !mask[b]) return;
   int jt0, jt1;
   group_tiles(T0, T1, w % ngroups, ngroups, jt0, jt1);
   if (jt0 >= jt1) return;
@@ -769,10 +773,12 @@ __global__ __launch_bounds__(256 * NBANDS) __attribute__((amdgpu_waves_per_eu(2,
 template <int KB, typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gj_update_one(
     const T* __restrict__ X, T* __restrict__ Out, long long ostride, int ldo, int ocol0, int Np, int k0,
-    const T* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0) {
+    const T* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0,
+    const int* __restrict__ mask) {
   __shared__ T sB[1][KB][64];
   int b, w;
   xcd_matrix_map(b, w);
+  if (mask && !mask[b]) return;
   gj_update_body<KB, 1, true, T>(X, Out, ostride, ldo, ocol0, Np, k0, AT, src2, brow, b, w, T0, T0 + 1, threadIdx.x, sB);
 }
 
@@ -788,6 +794,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const T* __restrict__ AT, const int* __restrict__ src2, const int* __restrict__ brow, int T0, int T1,
     int ngroups) {
   if ((int)blockIdx.x < npanel) {
+    if (P.mask && !P.mask[blockIdx.x]) return;
     gj_pair_chain_body<RPT, T>(P, blockIdx.x);
     return;
   }
@@ -807,6 +814,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     b = u / per_b;
     rem = u % per_b;
   }
+  if (P.mask && !P.mask[b]) return;
   int jt0, jt1;
   group_tiles(T0, T1, rem % ngroups, ngroups, jt0, jt1);
   if (jt0 >= jt1) return;
@@ -818,8 +826,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // out[b][r][c] = in[b][r][pinv[c]], converted to the output type.
 template <typename Tin, typename Tout>
 __global__ void k_unpermute(const Tin* __restrict__ in, Tout* __restrict__ out, int Np,
-                            const int* __restrict__ pinv) {
+                            const int* __restrict__ pinv, const int* __restrict__ mask) {
   const int b = blockIdx.y;
+  if (mask && !mask[b]) return;
   const size_t n2 = (size_t)Np * Np;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n2; idx += (size_t)gridDim.x * blockDim.x) {
     const int r = (int)(idx / Np);
@@ -903,10 +912,10 @@ thread_local int t_walk_update = 0;
 
 template <typename T>
 int launch_panel(const T* X, long long bstride, int ld, int col0, int batch, int Np, int k0, T* pout,
-                 int* src, int* piv, int* perm, int* pinv, const PanelMaps<T>& m, hipStream_t stream) {
+                 int* src, int* piv, int* perm, int* pinv, const PanelMaps<T>& m, const int* mask, hipStream_t stream) {
 #define TICC_PANEL_ARGS                                                                                          \
   X, bstride, ld, col0, Np, k0, pout, src, piv, perm, pinv, m.PoutT, m.tstride, m.prev_src, m.prev_piv, m.prev_k0, \
-      m.src2, m.brow
+      m.src2, m.brow, mask
   if (Np <= 256)
     hipLaunchKernelGGL((k_gj_panel<256, 1, T>), dim3(batch), dim3(256), 0, stream, TICC_PANEL_ARGS);
   else if (Np <= 512)
@@ -922,7 +931,8 @@ int launch_panel(const T* X, long long bstride, int ld, int col0, int batch, int
 
 template <int KB, typename T>
 int launch_update(const T* cur, T* out, long long ostride, int ldo, int ocol0, int batch, int Np, int k0,
-                  const T* AT, const int* src2, const int* brow, int T0, int T1, hipStream_t stream) {
+                  const T* AT, const int* src2, const int* brow, int T0, int T1, const int* mask,
+                  hipStream_t stream) {
   // standalone launches run one band per workgroup (faster alone than the
   // two-band form, which pays off beside a panel in k_gj_fused)
   constexpr int NBANDS = 1;
@@ -930,7 +940,7 @@ int launch_update(const T* cur, T* out, long long ostride, int ldo, int ocol0, i
   const int span = T1 - T0;
   if (span == 1) {
     hipLaunchKernelGGL((k_gj_update_one<KB, T>), dim3(units, batch), dim3(256), 0, stream, cur, out, ostride, ldo, ocol0,
-                       Np, k0, AT, src2, brow, T0);
+                       Np, k0, AT, src2, brow, T0, mask);
     TICC_CHECK_LAUNCH();
     return TICC_OK;
   }
@@ -940,7 +950,7 @@ int launch_update(const T* cur, T* out, long long ostride, int ldo, int ocol0, i
   if (ngroups > span) ngroups = span;
   if (ngroups < 1) ngroups = 1;
   hipLaunchKernelGGL((k_gj_update<KB, NBANDS, T>), dim3(units * ngroups, batch), dim3(256 * NBANDS), 0, stream, cur, out,
-                     ostride, ldo, ocol0, Np, k0, AT, src2, brow, T0, T1, ngroups);
+                     ostride, ldo, ocol0, Np, k0, AT, src2, brow, T0, T1, ngroups, mask);
   TICC_CHECK_LAUNCH();
   return TICC_OK;
 }
@@ -976,8 +986,9 @@ Carve<T> carve(T* pout, int* src, int* piv, int batch, int Np) {
 
 template <typename T>
 PairJob<T> pair_job(const T* Xc, long long xs, int ldx, int xcol, int Np, int k0, const Carve<T>& c, int par,
-                    int* perm, int* pinv) {
+                    int* perm, int* pinv, const int* mask) {
   PairJob<T> P;
+  P.mask = mask;
   P.Xc = Xc;
   P.xs = xs;
   P.ldx = ldx;
@@ -1014,15 +1025,16 @@ int launch_pair_chain(const PairJob<T>& P, int batch, hipStream_t stream) {
 
 template <typename T>
 int single_step(T* cur, T* nxt, int batch, int Np, int k0, const Carve<T>& c, int* perm, int* pinv,
-                hipStream_t st) {
+                const int* mask, hipStream_t st) {
   const long long n2 = (long long)Np * Np;
   PanelMaps<T> m;
   m.PoutT = c.AT[0];
   m.tstride = 2LL * NB * Np;
   m.src2 = c.src2[0];
   m.brow = c.brow[0];
-  TICC_RETURN_IF(launch_panel(cur, n2, Np, k0, batch, Np, k0, c.P1, c.s1, c.p1, perm, pinv, m, st));
-  return launch_update<NB, T>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[0], c.src2[0], c.brow[0], 0, (Np + 63) / 64, st);
+  TICC_RETURN_IF(launch_panel(cur, n2, Np, k0, batch, Np, k0, c.P1, c.s1, c.p1, perm, pinv, m, mask, st));
+  return launch_update<NB, T>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[0], c.src2[0], c.brow[0], 0, (Np + 63) / 64,
+                              mask, st);
 }
 
 // Fused launch: the panel chain P beside column tiles [T0, T1) of the rank-2NB
@@ -1066,7 +1078,7 @@ int launch_fused(const PairJob<T>& P, const T* X, T* Out, int batch, int Np, int
 
 template <typename T>
 int gj_invert_t(T* bufA, T* bufB, int batch, int Np, T** result, int* perm, int* pinv, T* pout, int* src, int* piv,
-                hipStream_t stream) {
+                const int* mask, hipStream_t stream) {
   if (Np % NB != 0 || Np > MAX_NP || Np <= 0 || batch <= 0)
     return fail(TICC_ERR_VALUE, "gj_invert_raw: invalid Np=%d batch=%d", Np, batch);
   const Carve<T> c = carve<T>(pout, src, piv, batch, Np);
@@ -1080,7 +1092,7 @@ int gj_invert_t(T* bufA, T* bufB, int batch, int Np, T** result, int* perm, int*
   // the lock so that inversions on concurrent lanes never share a slot.
   const int n_fused = pairs > 0 ? pairs - 1 + (steps & 1) : 0;
   long long slot = -1;
-  if (g_prof.on && n_fused > 0) {
+  if (g_prof.on && n_fused > 0 && !mask) {  // masked (fallback) inversions are not timed
     std::lock_guard<std::mutex> lock(g_prof.mu);
     if (g_prof.on && (g_prof.calls++ % g_prof.every == 0) && g_prof.used + (size_t)n_fused <= g_prof.bytes.size()) {
       slot = (long long)g_prof.used;
@@ -1089,7 +1101,7 @@ int gj_invert_t(T* bufA, T* bufB, int batch, int Np, T** result, int* perm, int*
   }
   auto next_slot = [&]() { return slot < 0 ? -1LL : slot++; };
   const long long ln = 2LL * NB * Np;  // look-ahead buffer: Np x 2NB per matrix
-  if (pairs > 0) TICC_RETURN_IF(launch_pair_chain(pair_job<T>(buf[0], n2, Np, 0, Np, 0, c, 0, perm, pinv), batch, stream));
+  if (pairs > 0) TICC_RETURN_IF(launch_pair_chain(pair_job<T>(buf[0], n2, Np, 0, Np, 0, c, 0, perm, pinv, mask), batch, stream));
   for (int q = 0; q < pairs; ++q) {
     const int k0 = 2 * q * NB;
     const int par = q & 1;
@@ -1098,20 +1110,20 @@ int gj_invert_t(T* bufA, T* bufB, int batch, int Np, T** result, int* perm, int*
     if (q + 1 == pairs) {
       if (!(steps & 1)) {
         TICC_RETURN_IF(launch_update<2 * NB, T>(cur, nxt, n2, Np, 0, batch, Np, k0, c.AT[par], c.src2[par],
-                                             c.brow[par], 0, tiles, stream));
+                                             c.brow[par], 0, tiles, mask, stream));
         break;
       }
       // trailing single step: its panel (on a look-ahead of its columns) runs
       // inside the last pair's update, then the rank-NB update
       const int ks = k0 + 2 * NB;
       TICC_RETURN_IF(launch_update<2 * NB, T>(cur, c.L, ln, 2 * NB, ks, batch, Np, k0, c.AT[par], c.src2[par],
-                                           c.brow[par], ks / 64, ks / 64 + 1, stream));
-      PairJob<T> P = pair_job<T>(c.L, ln, 2 * NB, 0, Np, ks, c, par ^ 1, perm, pinv);
+                                           c.brow[par], ks / 64, ks / 64 + 1, mask, stream));
+      PairJob<T> P = pair_job<T>(c.L, ln, 2 * NB, 0, Np, ks, c, par ^ 1, perm, pinv, mask);
       P.single = 1;
       TICC_RETURN_IF(
           launch_fused(P, cur, nxt, batch, Np, k0, c.AT[par], c.src2[par], c.brow[par], 0, tiles, next_slot(), stream));
       TICC_RETURN_IF(launch_update<NB, T>(nxt, cur, n2, Np, 0, batch, Np, ks, c.AT[par ^ 1], c.src2[par ^ 1],
-                                       c.brow[par ^ 1], 0, tiles, stream));
+                                       c.brow[par ^ 1], 0, tiles, mask, stream));
       *result = cur;
       return TICC_OK;
     }
@@ -1119,13 +1131,13 @@ int gj_invert_t(T* bufA, T* bufB, int batch, int Np, T** result, int* perm, int*
     // its columns after pair q come from a one-tile look-ahead of update q.
     const int kn = k0 + 2 * NB;
     TICC_RETURN_IF(launch_update<2 * NB, T>(cur, c.L, ln, 2 * NB, kn, batch, Np, k0, c.AT[par], c.src2[par],
-                                         c.brow[par], kn / 64, kn / 64 + 1, stream));
-    TICC_RETURN_IF(launch_fused(pair_job<T>(c.L, ln, 2 * NB, 0, Np, kn, c, par ^ 1, perm, pinv), cur, nxt, batch, Np, k0,
+                                         c.brow[par], kn / 64, kn / 64 + 1, mask, stream));
+    TICC_RETURN_IF(launch_fused(pair_job<T>(c.L, ln, 2 * NB, 0, Np, kn, c, par ^ 1, perm, pinv, mask), cur, nxt, batch, Np, k0,
                                 c.AT[par], c.src2[par], c.brow[par], 0, tiles, next_slot(), stream));
   }
   T* cur = buf[pairs & 1];
   if (steps & 1) {  // trailing single step: rank-NB update
-    TICC_RETURN_IF(single_step(cur, buf[(pairs + 1) & 1], batch, Np, (steps - 1) * NB, c, perm, pinv, stream));
+    TICC_RETURN_IF(single_step(cur, buf[(pairs + 1) & 1], batch, Np, (steps - 1) * NB, c, perm, pinv, mask, stream));
     cur = buf[(pairs + 1) & 1];
   }
   *result = cur;
@@ -1136,12 +1148,17 @@ int gj_invert_t(T* bufA, T* bufB, int batch, int Np, T** result, int* perm, int*
 
 int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result, int* perm, int* pinv, double* pout,
                   int* src, int* piv, hipStream_t stream) {
-  return gj_invert_t<double>(bufA, bufB, batch, Np, result, perm, pinv, pout, src, piv, stream);
+  return gj_invert_t<double>(bufA, bufB, batch, Np, result, perm, pinv, pout, src, piv, nullptr, stream);
+}
+
+int gj_invert_raw_masked(double* bufA, double* bufB, int batch, int Np, double** result, int* perm, int* pinv,
+                         double* pout, int* src, int* piv, const int* mask, hipStream_t stream) {
+  return gj_invert_t<double>(bufA, bufB, batch, Np, result, perm, pinv, pout, src, piv, mask, stream);
 }
 
 int gj_invert_raw_f32(float* bufA, float* bufB, int batch, int Np, float** result, int* perm, int* pinv, float* pout,
                       int* src, int* piv, hipStream_t stream) {
-  return gj_invert_t<float>(bufA, bufB, batch, Np, result, perm, pinv, pout, src, piv, stream);
+  return gj_invert_t<float>(bufA, bufB, batch, Np, result, perm, pinv, pout, src, piv, nullptr, stream);
 }
 
 int gj_set_walk(int fused_target, int update_target) {
@@ -1154,26 +1171,31 @@ int gj_set_walk(int fused_target, int update_target) {
 
 namespace {
 template <typename Tin, typename Tout>
-int unpermute_t(const Tin* in, Tout* out, int batch, int Np, const int* pinv, hipStream_t stream) {
+int unpermute_t(const Tin* in, Tout* out, int batch, int Np, const int* pinv, const int* mask, hipStream_t stream) {
   const long long n2 = (long long)Np * Np;
   int blocks = (int)((n2 + 255) / 256);
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL((k_unpermute<Tin, Tout>), dim3(blocks, batch), dim3(256), 0, stream, in, out, Np, pinv);
+  hipLaunchKernelGGL((k_unpermute<Tin, Tout>), dim3(blocks, batch), dim3(256), 0, stream, in, out, Np, pinv, mask);
   TICC_CHECK_LAUNCH();
   return TICC_OK;
 }
 }  // namespace
 
 int gj_unpermute(const double* in, double* out, int batch, int Np, const int* pinv, hipStream_t stream) {
-  return unpermute_t<double, double>(in, out, batch, Np, pinv, stream);
+  return unpermute_t<double, double>(in, out, batch, Np, pinv, nullptr, stream);
+}
+
+int gj_unpermute_masked(const double* in, double* out, int batch, int Np, const int* pinv, const int* mask,
+                        hipStream_t stream) {
+  return unpermute_t<double, double>(in, out, batch, Np, pinv, mask, stream);
 }
 
 int gj_unpermute_f32(const float* in, double* out, int batch, int Np, const int* pinv, hipStream_t stream) {
-  return unpermute_t<float, double>(in, out, batch, Np, pinv, stream);
+  return unpermute_t<float, double>(in, out, batch, Np, pinv, nullptr, stream);
 }
 
 int gj_unpermute_f32f32(const float* in, float* out, int batch, int Np, const int* pinv, hipStream_t stream) {
-  return unpermute_t<float, float>(in, out, batch, Np, pinv, stream);
+  return unpermute_t<float, float>(in, out, batch, Np, pinv, nullptr, stream);
 }
 
 }  // namespace ticc

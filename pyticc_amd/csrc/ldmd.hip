@@ -13,6 +13,7 @@
 // The elementwise passes fuse the column unpermute of the GJ result.
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 
 #include "ticc_common.h"
 
@@ -304,11 +305,31 @@ __global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out,
     out[i] = (float)in[i];
 }
 
+__global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// mask[b] = 1 where the FP32 inverse's residual max|I - A X| exceeds tau (its
+// Newton steps would not reach FP64 accuracy): that matrix is inverted in FP64.
+__global__ void k_flag(const unsigned* __restrict__ rho, int n, unsigned tau_bits, int* __restrict__ mask,
+                       unsigned long long* __restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int f = rho[i] > tau_bits ? 1 : 0;
+  mask[i] = f;
+  if (f) atomicAdd(count, 1ull);
+}
+
 // The identity column map (a plain inverse needs no unpermute).
 __global__ void k_iota(int* __restrict__ p, int Np, long long n) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = (int)(i % Np);
 }
+
+// Counters of the calling thread's mixed-precision sweeps (see mixed_stats).
+thread_local long long t_mixed_inversions = 0;
+thread_local long long t_mixed_fallbacks = 0;
 
 int grid1(long long n) {
   long long b = (n + 255) / 256;
@@ -317,6 +338,13 @@ int grid1(long long n) {
 }
 
 }  // namespace
+
+int mixed_stats(long long* inversions, long long* fallbacks, int reset) {
+  *inversions = t_mixed_inversions;
+  *fallbacks = t_mixed_fallbacks;
+  if (reset) t_mixed_inversions = t_mixed_fallbacks = 0;
+  return TICC_OK;
+}
 
 int to_f32(const double* in, float* out, long long n, hipStream_t stream) {
   hipLaunchKernelGGL(k_to_f32, dim3(grid1(n)), dim3(256), 0, stream, in, out, n);
@@ -382,6 +410,7 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   const size_t oE = take(sizeof(double) * nE);
   const size_t oH = take(sizeof(double) * nS);
   const size_t oFlag = take(sizeof(int));
+  const size_t oCount = take(sizeof(unsigned long long));  // mixed sweep: matrices inverted in FP64
   // refine >= 0: the mixed-precision sweep -- each sector matrix is inverted
   // in FP32 and refined by `refine` Newton steps X <- X + X (I - A X), the
   // residual I - A X on the FP64 MFMA from the FP64 A, the correction X (I - A X)
@@ -390,7 +419,7 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   const bool mixed = refine >= 0;
   struct Scratch {
     size_t A0, A1, P, src, piv, perm, pinv, y1, y2;
-    size_t F0, F1, X, T32, X32[2], ipinv;  // mixed sweep only
+    size_t F0, F1, X, T32, X32[2], ipinv, XB, rho, mask;  // mixed sweep only
   };
   auto scratch = [&](int batch, bool yv) {
     Scratch sc{};
@@ -402,6 +431,9 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
       sc.X32[0] = take(sizeof(float) * batch * n2);
       sc.X32[1] = take(sizeof(float) * batch * n2);
       sc.ipinv = take(sizeof(int) * batch * Np);
+      sc.XB = take(sizeof(double) * batch * n2);
+      sc.rho = take(sizeof(unsigned) * batch);
+      sc.mask = take(sizeof(int) * batch);
     }
     sc.A0 = yv ? take(sizeof(double) * batch * n2) : 0;
     sc.A1 = yv ? take(sizeof(double) * batch * n2) : 0;
@@ -442,6 +474,12 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   TICC_CHECK_HIP(hipMemcpyAsync(dE, E_host, sizeof(double) * nE, hipMemcpyHostToDevice, stream));
   TICC_CHECK_HIP(hipMemcpyAsync(dH, hs_host, sizeof(double) * nS, hipMemcpyHostToDevice, stream));
   TICC_CHECK_HIP(hipMemsetAsync(flag, 0, sizeof(int), stream));
+  unsigned long long* fb_count = reinterpret_cast<unsigned long long*>(w + oCount);
+  if (mixed) TICC_CHECK_HIP(hipMemsetAsync(fb_count, 0, sizeof(unsigned long long), stream));
+  float tau = 1.0e-4f;  // residual bound for keeping the refined FP32 inverse
+  if (const char* env = getenv("TICC_MIXED_FALLBACK_TAU")) tau = (float)atof(env);  // tuning knob
+  unsigned tau_bits = 0;
+  memcpy(&tau_bits, &tau, sizeof(tau));
   TICC_RETURN_IF(pad_identity(Y, (long long)N * N, N, Yp, Np, nE, stream));
   if (mixed)
     for (int g = 0; g < G; ++g) {
@@ -507,11 +545,29 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
                                          reinterpret_cast<float*>(D(sc.P)), I(sc.src), I(sc.piv), st));
         TICC_RETURN_IF(gj_unpermute_f32(r32, D(sc.X), ne, Np, I(sc.pinv), st));
         if (refine > 0) TICC_RETURN_IF(gj_unpermute_f32f32(r32, F(sc.X32[0]), ne, Np, I(sc.pinv), st));
+        unsigned* rho = reinterpret_cast<unsigned*>(w + sc.rho);
+        int* mask = I(sc.mask);
         for (int it = 0; it < refine; ++it) {
-          // T = I - A X (FP64 operands, FP32 result), X <- X + X T (FP32 operands)
-          TICC_RETURN_IF(refine_gemm_f64(Np, ne, Ain, D(sc.X), -1.0, nullptr, 0.0, 1.0, nullptr, F(sc.T32), st));
+          // T = I - A X (FP64 operands, FP32 result), X <- X + X T (FP32 operands);
+          // the first residual's largest element decides which matrices fall back
+          if (it == 0) {
+            hipLaunchKernelGGL(k_fill_u32, dim3((ne + 255) / 256), dim3(256), 0, st, rho, ne, 0u);
+            TICC_CHECK_LAUNCH();
+          }
+          TICC_RETURN_IF(refine_gemm_f64(Np, ne, Ain, D(sc.X), -1.0, nullptr, 0.0, 1.0, nullptr, F(sc.T32),
+                                         it == 0 ? rho : nullptr, st));
+          if (it == 0) {
+            hipLaunchKernelGGL(k_flag, dim3((ne + 255) / 256), dim3(256), 0, st, rho, ne, tau_bits, mask, fb_count);
+            TICC_CHECK_LAUNCH();
+          }
           TICC_RETURN_IF(refine_gemm_f32(Np, ne, F(sc.X32[it & 1]), F(sc.T32), 1.0, D(sc.X), 1.0, 0.0, D(sc.X),
                                          it + 1 < refine ? F(sc.X32[(it + 1) & 1]) : nullptr, st));
+        }
+        if (refine > 0) {  // the flagged matrices: FP64 inversion of A in place (every kernel skips the others)
+          double* rf = nullptr;
+          TICC_RETURN_IF(gj_invert_raw_masked(Ain, D(sc.XB), ne, Np, &rf, I(sc.perm), I(sc.pinv), D(sc.P), I(sc.src),
+                                              I(sc.piv), mask, st));
+          TICC_RETURN_IF(gj_unpermute_masked(rf, D(sc.X), ne, Np, I(sc.pinv), mask, st));
         }
         *X = D(sc.X);
         *pv = I(sc.ipinv);
@@ -568,8 +624,14 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   TICC_RETURN_IF(unpad(Yp, Np, Y, (long long)N * N, N, nE, stream));
   TICC_RETURN_IF(check_finite(Y, (long long)nE * N * N, flag, stream));
   int hflag = 0;
+  unsigned long long hcount = 0;
   TICC_CHECK_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, stream));
+  if (mixed) TICC_CHECK_HIP(hipMemcpyAsync(&hcount, fb_count, sizeof(hcount), hipMemcpyDeviceToHost, stream));
   TICC_CHECK_HIP(hipStreamSynchronize(stream));
+  if (mixed) {
+    t_mixed_inversions += 2LL * nS * nE;
+    t_mixed_fallbacks += (long long)hcount;
+  }
   if (hflag) return fail(TICC_ERR_NONFINITE, "LDMD sweep produced a non-finite log-derivative matrix");
   return TICC_OK;
 }

@@ -108,11 +108,18 @@ namespace ticc {
 // the column of *result that holds column c of the inverse.
 int gj_invert_raw(double* bufA, double* bufB, int batch, int Np, double** result, int* perm, int* pinv,
                   double* pout, int* src, int* piv, hipStream_t stream);
+// The same inversion restricted to the matrices b with mask[b] != 0 (device
+// array): every kernel's workgroups of the other matrices exit at once.
+int gj_invert_raw_masked(double* bufA, double* bufB, int batch, int Np, double** result, int* perm, int* pinv,
+                         double* pout, int* src, int* piv, const int* mask, hipStream_t stream);
 // The same inversion in FP32 (FP32 panels, FP32 MFMA updates, same pivot rule).
 int gj_invert_raw_f32(float* bufA, float* bufB, int batch, int Np, float** result, int* perm, int* pinv, float* pout,
                       int* src, int* piv, hipStream_t stream);
 // Gather the column permutation into a plain inverse: out[b][r][c] = in[b][r][pinv[c]].
 int gj_unpermute(const double* in, double* out, int batch, int Np, const int* pinv, hipStream_t stream);
+// The gather for the matrices with mask[b] != 0 only (the others keep `out`).
+int gj_unpermute_masked(const double* in, double* out, int batch, int Np, const int* pinv, const int* mask,
+                        hipStream_t stream);
 // The same gather from an FP32 inverse into FP64.
 int gj_unpermute_f32(const float* in, double* out, int batch, int Np, const int* pinv, hipStream_t stream);
 int gj_unpermute_f32f32(const float* in, float* out, int batch, int Np, const int* pinv, hipStream_t stream);
@@ -122,9 +129,10 @@ int dgemm_batched(bool transA, bool transB, int m, int n, int k, double alpha, c
                   long long strideC, int batch, hipStream_t stream);
 // Batched square GEMMs of the Newton refinement (Np a multiple of 32):
 // out = beta Cin + alpha A B + diag I, out32 = (float) out; either output may be
-// null.  FP64 operands on the FP64 MFMA, or FP32 operands on the FP32 MFMA.
+// null.  FP64 operands on the FP64 MFMA, or FP32 operands on the FP32 MFMA.  The
+// FP64 form can raise rho[b] (float bits) to the largest |out[b]| element.
 int refine_gemm_f64(int Np, int batch, const double* A, const double* B, double alpha, const double* Cin, double beta,
-                    double diag, double* out, float* out32, hipStream_t stream);
+                    double diag, double* out, float* out32, unsigned* rho, hipStream_t stream);
 int refine_gemm_f32(int Np, int batch, const float* A, const float* B, double alpha, const double* Cin, double beta,
                     double diag, double* out, float* out32, hipStream_t stream);
 // FP32 copy of n doubles.

@@ -38,6 +38,9 @@ def test_headline_bench_line(gpu_device):
     assert 0 < d["sweep"]["frac_of_fp64_peak"] < 1
     assert d["parity"] is None  # a reduced run names no fixture comparison
     assert d["max_unitarity_defect"] < 1e-8
+    m = d["mixed"]
+    assert m["value"] > 0 and m["refine_steps"] == 2 and m["ratio_to_fp64"] > 0
+    assert m["max_abs_dS_vs_fp64_run"] <= 1e-6 and m["max_unitarity_defect"] < 1e-8
 
 
 def test_batching_bench_line(gpu_device):

@@ -97,6 +97,39 @@ def test_mixed_sweep_matches_oracle(gpu_device, N, n_E, n_S):
     np.testing.assert_array_equal(got, np.swapaxes(got, 1, 2))
 
 
+def _mixed_stats(reset=True):
+    inv = np.zeros(1, dtype=np.int64)
+    fb = np.zeros(1, dtype=np.int64)
+    _backend.call("ticc_mixed_stats", inv.ctypes.data, fb.ctypes.data, int(reset))
+    return int(inv[0]), int(fb[0])
+
+
+def test_mixed_sweep_with_every_matrix_falling_back_is_the_fp64_sweep(gpu_device, monkeypatch):
+    """A sector matrix whose refined FP32 inverse would miss FP64 accuracy is
+    inverted in FP64 instead; forcing that for every matrix (residual bound 0)
+    must reproduce the FP64 sweep bit for bit."""
+    from pyticc_amd.propagation.logd import propagate_logD
+
+    rng = np.random.default_rng(17)
+    N, n_E, n_S = 96, 3, 6
+    Ws, Wm, We = random_W_batch(rng, n_S, N, scale=20.0)
+    We[:-1] = Ws[1:]
+    hs = rng.uniform(0.01, 0.03, n_S)
+    E = rng.uniform(0.5, 2.0, n_E)
+    mu = 3.0
+    Y0 = oldmd.initialize_inelastic(Ws[0][None] - 2 * mu * E[:, None, None] * np.eye(N))
+    ref = _device.to_host(propagate_logD(Y0, E, mu, hs, Ws, Wm, We))
+    _mixed_stats(reset=True)
+    monkeypatch.setenv("TICC_MIXED_FALLBACK_TAU", "0")
+    got = _sweep_mixed(gpu_device, Y0, E, mu, hs, Ws, Wm, We, 2)
+    assert _mixed_stats() == (2 * n_S * n_E, 2 * n_S * n_E)
+    np.testing.assert_array_equal(got, ref)
+    monkeypatch.setenv("TICC_MIXED_FALLBACK_TAU", "1e30")  # never: the refined FP32 inverses throughout
+    got = _sweep_mixed(gpu_device, Y0, E, mu, hs, Ws, Wm, We, 2)
+    assert _mixed_stats() == (2 * n_S * n_E, 0)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-10 * np.abs(ref).max())
+
+
 def _cross_section_change(S, S_ref, res):
     """max over energies and level pairs of |sigma - sigma_ref| / max(sigma_ref) at that energy."""
     from dataclasses import replace
@@ -144,6 +177,7 @@ def test_mixed_solve_rotor_rotor_N800_full_size_matches_fixture(gpu_device):
     w = synthetic.rotor_rotor()
     idx = d["energy_index"]
     grid = ticc.prepare_potential(w.system, w.boundaries, w.half_steps, **w.quadrature)
+    _mixed_stats(reset=True)
     res = ticc.solve(w.system, w.energies[idx], grid,
                      ticc.Propagation(device="gpu", memory_mb=4096.0, precision="mixed", refine_steps=2))
     report = []
@@ -154,7 +188,8 @@ def test_mixed_solve_rotor_rotor_N800_full_size_matches_fixture(gpu_device):
             continue
         r, c, ref = rotor_rotor_N800_reference(d, i)
         report.append((int(idx[i]), S.shape[0], float(np.max(np.abs(S[r, c] - ref))), omatch.unitarity_defect(S)))
-    print("N = 800 mixed (energy, n_open, max|dS|, unitarity defect):", report)
+    print("N = 800 mixed (energy, n_open, max|dS|, unitarity defect):", report, "inversions, FP64 fallbacks:",
+          _mixed_stats())
     for e, n, err, defect in report:
         assert err <= TOL_MIXED, (e, n, err)
         assert defect < 1e-9, (e, defect)
