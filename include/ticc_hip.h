@@ -150,7 +150,7 @@ int ticc_ldmd_propagate_blocks(int n_block, int n_energy, int n_channel, const d
  * X <- X + X (I - A X): the residual I - A X from the FP64 A on the FP64 MFMA,
  * the correction on the FP32 MFMA.  Reference values, corrections, Q_mid, the
  * form steps and Y stay FP64.  A matrix whose first residual max|I - A X|
- * exceeds 1e-4 (TICC_MIXED_FALLBACK_TAU) is inverted in FP64 instead, so
+ * exceeds 1e-3 (TICC_MIXED_FALLBACK_TAU) is inverted in FP64 instead, so
  * ill-conditioned sector matrices near log-derivative poles keep FP64 accuracy.
  * Two steps keep the S matrix within 1e-6 of the FP64 sweep on the benchmark
  * systems.
@@ -162,7 +162,7 @@ int ticc_ldmd_propagate_mixed(int n_energy, int n_channel, const double* energie
 /*
  * Counters of the calling thread's mixed-precision sweeps since the last reset:
  * the sector-matrix inversions done, and how many of them were redone in FP64
- * because the FP32 inverse's residual max|I - A X| exceeded 1e-4 (near a
+ * because the FP32 inverse's residual max|I - A X| exceeded 1e-3 (near a
  * log-derivative pole the sector matrix is too ill-conditioned for FP32
  * refinement to reach FP64 accuracy).  reset != 0 zeroes them after reading.
  */

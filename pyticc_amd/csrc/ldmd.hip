@@ -476,7 +476,7 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   TICC_CHECK_HIP(hipMemsetAsync(flag, 0, sizeof(int), stream));
   unsigned long long* fb_count = reinterpret_cast<unsigned long long*>(w + oCount);
   if (mixed) TICC_CHECK_HIP(hipMemsetAsync(fb_count, 0, sizeof(unsigned long long), stream));
-  float tau = 1.0e-4f;  // residual bound for keeping the refined FP32 inverse
+  float tau = 1.0e-3f;  // residual bound for keeping the refined FP32 inverse
   if (const char* env = getenv("TICC_MIXED_FALLBACK_TAU")) tau = (float)atof(env);  // tuning knob
   unsigned tau_bits = 0;
   memcpy(&tau_bits, &tau, sizeof(tau));
