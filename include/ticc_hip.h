@@ -169,6 +169,40 @@ int ticc_ldmd_propagate_mixed(int n_energy, int n_channel, const double* energie
 int ticc_mixed_stats(long long* inversions, long long* fallbacks, int reset);
 
 /*
+ * Enqueue-only form of ticc_ldmd_propagate_blocks, for capture into a HIP graph
+ * (the batching form: many blocks' sweeps replayed as one graph launch; the
+ * reference's batched block loop is src/pyticc/propagation/runner.py:326-374).
+ * It performs no host-device copy, memset, allocation, host synchronisation or
+ * finite check:
+ *   energies_dev    device double[n_block * n_energy], block-major
+ *   half_steps      host double[n_sector] (its values are baked into the kernels'
+ *                   arguments), half_steps_dev the same values on the device
+ *   Y_in            device initial Y (n_block * n_energy, N, N); Y receives the
+ *                   result, so a replayed graph restarts from Y_in every time
+ *   scratch         caller-owned device memory of *scratch_bytes bytes; with
+ *                   scratch == NULL only *scratch_bytes is set (the size needed)
+ *                   and nothing is enqueued.
+ * FP64 inelastic sweep; the caller checks Y for non-finite values.
+ */
+int ticc_ldmd_propagate_blocks_async(int n_block, int n_energy, int n_channel, double reduced_mass, int n_sector,
+                                     const double* half_steps, const double* energies_dev, const double* half_steps_dev,
+                                     const double* W_start, const double* W_mid, const double* W_end,
+                                     long long w_sector_stride, long long w_block_stride, const double* Y_in, double* Y,
+                                     void* scratch, long long* scratch_bytes, void* stream);
+
+/*
+ * Stream capture into a HIP graph (no counterpart in the reference): begin
+ * (relaxed capture mode) on a non-default stream, enqueue work (e.g. the sweeps
+ * above), end -> an instantiated graph handle; launch replays it on a stream;
+ * destroy frees it.  Work captured must not copy from pageable host memory or
+ * allocate; device buffers it uses must outlive the graph.
+ */
+int ticc_graph_begin(void* stream);
+int ticc_graph_end(void* stream, void** graph);
+int ticc_graph_launch(void* graph, void* stream);
+int ticc_graph_destroy(void* graph);
+
+/*
  * Capture boundary condition (Propagation(mode="capture")): complex symmetric
  * log derivatives Z carried in the real form [[Re Z, -Im Z], [Im Z, Re Z]]
  * (2N x 2N), on which sums, products, inverses and real diagonal shifts act as

@@ -45,6 +45,14 @@ _SIGNATURES = {
         [_c_int, _c_int, _c_int, _dp, _c_d, _c_int, _dp, _vp, _vp, _vp, _c_ll, _c_ll, _vp, _vp],
         _c_int,
     ),
+    "ticc_ldmd_propagate_blocks_async": (
+        [_c_int, _c_int, _c_int, _c_d, _c_int, _dp, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _vp, _vp, _vp, _vp, _vp],
+        _c_int,
+    ),
+    "ticc_graph_begin": ([_vp], _c_int),
+    "ticc_graph_end": ([_vp, _vp], _c_int),
+    "ticc_graph_launch": ([_vp, _vp], _c_int),
+    "ticc_graph_destroy": ([_vp], _c_int),
     "ticc_ldmd_init_capture": ([_c_int, _c_int, _vp, _dp, _c_d, _vp, _vp], _c_int),
     "ticc_ldmd_propagate_capture": (
         [_c_int, _c_int, _dp, _c_d, _c_int, _dp, _vp, _vp, _vp, _c_ll, _vp, _vp],

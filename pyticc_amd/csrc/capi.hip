@@ -24,9 +24,7 @@ int mlp_colsum(int M, int N, const float* X, float* out, hipStream_t stream);
 int mlp_adam(int rows, int cols, float* theta, const float* grad, float* m, float* v, float lr, float b1, float b2,
              float eps, int step, void* bf, void* bfT, hipStream_t stream);
 int mlp_to_bf16(int rows, int cols, const float* x, void* bf, void* bfT, hipStream_t stream);
-int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const double* hs_host, const double* Ws0,
-                   const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream, int half,
-                   int n_block, long long gstride, int refine);
+
 int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, const double* E_host, double mu,
              const double* Eint_host, const double* L_host, double* S_re, double* S_im, int* n_open,
              hipStream_t stream);
@@ -187,7 +185,7 @@ int ticc_ldmd_propagate_capture(int n_energy, int n_channel, const double* energ
   TICC_RETURN_IF(sweep_args(n_energy, n_sector, energies, half_steps, W_start, W_mid, W_end, Y));
   if (n_sector == 0) return TICC_OK;
   return ticc::ldmd_propagate(n_energy, 2 * n_channel, energies, reduced_mass, n_sector, half_steps, W_start, W_mid,
-                              W_end, w_sector_stride, Y, as_stream(stream), n_channel, 1, 0, -1);
+                              W_end, w_sector_stride, Y, as_stream(stream), n_channel, 1, 0, -1, nullptr);
 }
 
 int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, double reduced_mass, int n_sector,
@@ -197,7 +195,7 @@ int ticc_ldmd_propagate(int n_energy, int n_channel, const double* energies, dou
   TICC_RETURN_IF(sweep_args(n_energy, n_sector, energies, half_steps, W_start, W_mid, W_end, Y));
   if (n_sector == 0) return TICC_OK;
   return ticc::ldmd_propagate(n_energy, n_channel, energies, reduced_mass, n_sector, half_steps, W_start, W_mid, W_end,
-                              w_sector_stride, Y, as_stream(stream), 0, 1, 0, -1);
+                              w_sector_stride, Y, as_stream(stream), 0, 1, 0, -1, nullptr);
 }
 
 int ticc_ldmd_propagate_mixed(int n_energy, int n_channel, const double* energies, double reduced_mass, int n_sector,
@@ -209,7 +207,7 @@ int ticc_ldmd_propagate_mixed(int n_energy, int n_channel, const double* energie
     return fail(TICC_ERR_VALUE, "refine_steps must lie in [0, 3], got %d", refine_steps);
   if (n_sector == 0) return TICC_OK;
   return ticc::ldmd_propagate(n_energy, n_channel, energies, reduced_mass, n_sector, half_steps, W_start, W_mid, W_end,
-                              w_sector_stride, Y, as_stream(stream), 0, 1, 0, refine_steps);
+                              w_sector_stride, Y, as_stream(stream), 0, 1, 0, refine_steps, nullptr);
 }
 
 int ticc_ldmd_propagate_blocks(int n_block, int n_energy, int n_channel, const double* energies, double reduced_mass,
@@ -225,7 +223,76 @@ int ticc_ldmd_propagate_blocks(int n_block, int n_energy, int n_channel, const d
   for (int b = 0; b < n_block; ++b)
     for (int e = 0; e < n_energy; ++e) all[(size_t)b * n_energy + e] = energies[e];
   return ticc::ldmd_propagate(n_block * n_energy, n_channel, all.data(), reduced_mass, n_sector, half_steps, W_start,
-                              W_mid, W_end, w_sector_stride, Y, as_stream(stream), 0, n_block, w_block_stride, -1);
+                              W_mid, W_end, w_sector_stride, Y, as_stream(stream), 0, n_block, w_block_stride, -1, nullptr);
+}
+
+int ticc_ldmd_propagate_blocks_async(int n_block, int n_energy, int n_channel, double reduced_mass, int n_sector,
+                                     const double* half_steps, const double* energies_dev, const double* half_steps_dev,
+                                     const double* W_start, const double* W_mid, const double* W_end,
+                                     long long w_sector_stride, long long w_block_stride, const double* Y_in, double* Y,
+                                     void* scratch, long long* scratch_bytes, void* stream) {
+  TICC_RETURN_IF(check_channels(n_channel));
+  if (n_block < 1 || n_energy < 1 || n_sector < 1) return fail(TICC_ERR_VALUE, "n_block, n_energy, n_sector must be positive");
+  if ((long long)n_block * n_energy > (1LL << 20)) return fail(TICC_ERR_VALUE, "too many block-energy pairs");
+  if (!half_steps || !scratch_bytes) return fail(TICC_ERR_VALUE, "ldmd_propagate_blocks_async: null pointer argument");
+  for (int s = 0; s < n_sector; ++s)
+    if (!(half_steps[s] > 0.0) || !std::isfinite(half_steps[s]))
+      return fail(TICC_ERR_VALUE, "radial_half_step must be positive, but got radial_half_step=%.17g", half_steps[s]);
+  if (scratch && (!energies_dev || !half_steps_dev || !W_start || !W_mid || !W_end || !Y_in || !Y))
+    return fail(TICC_ERR_VALUE, "ldmd_propagate_blocks_async: null pointer argument");
+  if (*scratch_bytes < 0) return fail(TICC_ERR_VALUE, "scratch_bytes must be non-negative");
+  ticc::SweepAsync as{energies_dev, half_steps_dev, Y_in, scratch, (size_t)*scratch_bytes};
+  TICC_RETURN_IF(ticc::ldmd_propagate(n_block * n_energy, n_channel, nullptr, reduced_mass, n_sector, half_steps,
+                                      W_start, W_mid, W_end, w_sector_stride, Y, as_stream(stream), 0, n_block,
+                                      w_block_stride, -1, &as));
+  *scratch_bytes = (long long)as.scratch_bytes;
+  return TICC_OK;
+}
+
+namespace {
+struct CapturedGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+}  // namespace
+
+int ticc_graph_begin(void* stream) {
+  if (!stream) return fail(TICC_ERR_VALUE, "graph capture needs a non-default stream");
+  TICC_CHECK_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeRelaxed));
+  return TICC_OK;
+}
+
+int ticc_graph_end(void* stream, void** graph) {
+  if (!stream || !graph) return fail(TICC_ERR_VALUE, "graph_end: null argument");
+  CapturedGraph* g = new CapturedGraph();
+  hipError_t e = hipStreamEndCapture(as_stream(stream), &g->graph);
+  if (e != hipSuccess) {
+    delete g;
+    return fail(TICC_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+  }
+  e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g->graph);
+    delete g;
+    return fail(TICC_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+  }
+  *graph = g;
+  return TICC_OK;
+}
+
+int ticc_graph_launch(void* graph, void* stream) {
+  if (!graph) return fail(TICC_ERR_VALUE, "graph_launch: null graph");
+  TICC_CHECK_HIP(hipGraphLaunch(static_cast<CapturedGraph*>(graph)->exec, as_stream(stream)));
+  return TICC_OK;
+}
+
+int ticc_graph_destroy(void* graph) {
+  if (!graph) return TICC_OK;
+  CapturedGraph* g = static_cast<CapturedGraph*>(graph);
+  if (g->exec) TICC_CHECK_HIP(hipGraphExecDestroy(g->exec));
+  if (g->graph) TICC_CHECK_HIP(hipGraphDestroy(g->graph));
+  delete g;
+  return TICC_OK;
 }
 
 int ticc_get_smat(int n_energy, int n_channel, const double* Y, const double* B, double Rmatch,

@@ -378,7 +378,7 @@ int ldmd_init_capture(int nE, int N, const double* W0, const double* E_host, dou
 
 int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const double* hs_host, const double* Ws0,
                    const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream, int half,
-                   int n_block, long long gstride, int refine) {
+                   int n_block, long long gstride, int refine, SweepAsync* as) {
   const int Np = round_up(N, GJ_NB);
   if (Np > MAX_NP) return fail(TICC_ERR_UNSUPPORTED, "n_channel=%d exceeds the supported maximum %d", N, MAX_NP);
   if (n_block < 1 || nE % n_block) return fail(TICC_ERR_VALUE, "the batch must hold the same energies for every block");
@@ -451,7 +451,18 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   Scratch sg[MAX_GROUPS];
   for (int g = 0; g < G; ++g) sg[g] = scratch(per, true);
   void* base = nullptr;
-  TICC_RETURN_IF(workspace(WS_SWEEP, off, &base));
+  if (as) {  // enqueue-only form: caller-owned scratch, device-resident inputs
+    if (refine >= 0 || half > 0) return fail(TICC_ERR_UNSUPPORTED, "the enqueue-only sweep is the FP64 inelastic sweep");
+    if (!as->scratch) {
+      as->scratch_bytes = off;
+      return TICC_OK;
+    }
+    if (as->scratch_bytes < off)
+      return fail(TICC_ERR_VALUE, "sweep scratch of %zu bytes is below the %zu bytes required", as->scratch_bytes, off);
+    base = as->scratch;
+  } else {
+    TICC_RETURN_IF(workspace(WS_SWEEP, off, &base));
+  }
   char* w = static_cast<char*>(base);
   auto D = [&](size_t o) { return reinterpret_cast<double*>(w + o); };
   auto I = [&](size_t o) { return reinterpret_cast<int*>(w + o); };
@@ -460,8 +471,8 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   double* Qm = D(oQ);
   double* M0 = D(oM);
   double* M1 = M0 + (size_t)n_block * qc * n2;
-  double* dE = D(oE);
-  double* dH = D(oH);
+  const double* dE = as ? as->E_dev : D(oE);
+  const double* dH = as ? as->hs_dev : D(oH);
   int* flag = I(oFlag);
 
   hipStream_t gs[MAX_GROUPS];
@@ -471,16 +482,18 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   hipEvent_t* ev_qm = ev;             // [2] Q_mid chunk ready (per buffer)
   hipEvent_t* ev_done = ev + 2;       // [G][2] group finished a chunk (per buffer)
 
-  TICC_CHECK_HIP(hipMemcpyAsync(dE, E_host, sizeof(double) * nE, hipMemcpyHostToDevice, stream));
-  TICC_CHECK_HIP(hipMemcpyAsync(dH, hs_host, sizeof(double) * nS, hipMemcpyHostToDevice, stream));
-  TICC_CHECK_HIP(hipMemsetAsync(flag, 0, sizeof(int), stream));
+  if (!as) {
+    TICC_CHECK_HIP(hipMemcpyAsync(D(oE), E_host, sizeof(double) * nE, hipMemcpyHostToDevice, stream));
+    TICC_CHECK_HIP(hipMemcpyAsync(D(oH), hs_host, sizeof(double) * nS, hipMemcpyHostToDevice, stream));
+    TICC_CHECK_HIP(hipMemsetAsync(flag, 0, sizeof(int), stream));
+  }
   unsigned long long* fb_count = reinterpret_cast<unsigned long long*>(w + oCount);
   if (mixed) TICC_CHECK_HIP(hipMemsetAsync(fb_count, 0, sizeof(unsigned long long), stream));
   float tau = 1.0e-3f;  // residual bound for keeping the refined FP32 inverse
   if (const char* env = getenv("TICC_MIXED_FALLBACK_TAU")) tau = (float)atof(env);  // tuning knob
   unsigned tau_bits = 0;
   memcpy(&tau_bits, &tau, sizeof(tau));
-  TICC_RETURN_IF(pad_identity(Y, (long long)N * N, N, Yp, Np, nE, stream));
+  TICC_RETURN_IF(pad_identity(as ? as->Y_in : Y, (long long)N * N, N, Yp, Np, nE, stream));
   if (mixed)
     for (int g = 0; g < G; ++g) {
       hipLaunchKernelGGL(k_iota, dim3(grid1((long long)per * Np)), dim3(256), 0, stream, I(sg[g].ipinv), Np,
@@ -622,6 +635,7 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   for (int g = 0; g < G; ++g)
     if (g * per < nE) TICC_CHECK_HIP(hipStreamWaitEvent(stream, ev_done[2 * g + last], 0));
   TICC_RETURN_IF(unpad(Yp, Np, Y, (long long)N * N, N, nE, stream));
+  if (as) return TICC_OK;  // no host synchronisation: the caller checks Y
   TICC_RETURN_IF(check_finite(Y, (long long)nE * N * N, flag, stream));
   int hflag = 0;
   unsigned long long hcount = 0;

@@ -135,6 +135,20 @@ int refine_gemm_f64(int Np, int batch, const double* A, const double* B, double 
                     double diag, double* out, float* out32, unsigned* rho, hipStream_t stream);
 int refine_gemm_f32(int Np, int batch, const float* A, const float* B, double alpha, const double* Cin, double beta,
                     double diag, double* out, float* out32, hipStream_t stream);
+// Enqueue-only form of the LDMD sweep (for stream capture): device-resident
+// energies (one per batch item) and half steps, the initial Y read from Y_in,
+// caller-owned scratch (scratch == null: only report the bytes needed), and no
+// host copy, memset, synchronisation or finite check.
+struct SweepAsync {
+  const double* E_dev;
+  const double* hs_dev;
+  const double* Y_in;
+  void* scratch;
+  size_t scratch_bytes;
+};
+int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const double* hs_host, const double* Ws0,
+                   const double* Wm0, const double* We0, long long wstride, double* Y, hipStream_t stream, int half,
+                   int n_block, long long gstride, int refine, SweepAsync* as);
 // FP32 copy of n doubles.
 int to_f32(const double* in, float* out, long long n, hipStream_t stream);
 // Identity padding: dst (batch, Np, Np) <- src (batch, N, N) with ld N, identity in the padding.

@@ -19,6 +19,7 @@ walk (``ticc_set_sweep_walk``) for each padded size.
 
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass, field
 from time import perf_counter
 
@@ -29,7 +30,9 @@ from pyticc_amd import _backend, _device
 from pyticc_amd._log import fail
 from pyticc_amd.energy import get_Etot
 from pyticc_amd.propagation.device import resolve_device
-from pyticc_amd.propagation.runner import propagate_blocks
+from pyticc_amd.propagation.grid import window_points
+from pyticc_amd.propagation.logd import initial_logD_energies
+from pyticc_amd.propagation.runner import block_sources, blocks_W_points, padding_diagonal, propagate_blocks
 
 PAD_QUANTUM = 32  # every internal matrix is padded to a multiple of the panel width
 DEFAULT_WALK = (512, 2048)  # (beside the panel chain, standalone) update workgroup targets
@@ -222,6 +225,111 @@ def propagate_buckets(hamiltonians, Etot, radial_sectors, config, plan, timings:
     if errors:
         raise errors[0]
     return out
+
+
+class CapturedBuckets:
+    """The FP64 sweeps of every bucket of a plan, captured once into one HIP graph
+    and replayed by one graph launch (the batching form: many uneven blocks x
+    energies through the radial grid without per-launch host work).
+
+    Construction contracts and pads every bucket's W_base over all sectors (one
+    radial window), forms the initial log derivatives, sizes one scratch buffer
+    for the largest bucket, runs each bucket's enqueue-only sweep once uncaptured,
+    and then captures them all, each with its bucket's update tile walk, on a
+    stream of its own.  Every input, output and scratch buffer the graph uses is
+    held by this object, and each replay restarts from the initial Y, so replays
+    are repeatable.  replay() enqueues one graph launch; results() gives each
+    block's (n_E, N_b, N_b) Y after it, in the caller's block order."""
+
+    def __init__(self, hamiltonians, Etot, radial_sectors, config, plan):
+        hams = list(hamiltonians)
+        if config.mode != "inelastic" or config.precision != "fp64":
+            raise fail(NotImplementedError, "the captured bucket sweep is the FP64 inelastic sweep")
+        self.sectors = tuple(radial_sectors)
+        if not self.sectors:
+            raise fail(RuntimeError, "Propagation produced no radial windows")
+        seen = sorted(i for b in plan for i in b.blocks)
+        if seen != list(range(len(hams))):
+            raise fail(ValueError, "the bucket plan must hold every block exactly once")
+        self.plan = tuple(plan)
+        self.dev = resolve_device(config.device).torch_device
+        E = np.ascontiguousarray(get_Etot(Etot), dtype=np.float64)
+        self.n_energy = E.size
+        mu = hams[0].reduced_mass
+        self.mu = float(mu)
+        pts = window_points(self.sectors)
+        self.hs = np.ascontiguousarray([s.radial_half_step for s in self.sectors], dtype=np.float64)
+        self.hs_dev = torch.as_tensor(self.hs).to(self.dev)
+        pad = padding_diagonal(mu, E)
+        self.sizes = [h.basis.n_channel for h in hams]
+        self.buckets = []
+        need = 0
+        for bucket in self.plan:
+            members = [hams[i] for i in bucket.blocks]
+            W = blocks_W_points(members, pts, self.dev, pad, block_sources(members))
+            nb, Nmax = W.shape[0], W.shape[-1]
+            Y0 = torch.cat([initial_logD_energies(W[b, 0], E, mu, self.dev) for b in range(nb)])
+            Edev = torch.as_tensor(np.tile(E, nb)).to(self.dev)
+            entry = {"bucket": bucket, "W": W, "Y0": Y0, "Y": torch.empty_like(Y0), "E": Edev, "N": Nmax, "nb": nb}
+            need = max(need, self._enqueue(entry, None, None))
+            self.buckets.append(entry)
+        self.scratch = torch.empty(max(need, 256), dtype=torch.uint8, device=self.dev)
+        self.stream = torch.cuda.Stream(device=self.dev)  # default priority
+        self.stream.wait_stream(torch.cuda.current_stream(self.dev))
+        self.graph = ctypes.c_void_p()
+        try:
+            for entry in self.buckets:  # uncaptured pass: library streams and events exist before the capture
+                _set_walk(entry["bucket"].walk)
+                self._enqueue(entry, self.scratch, self.stream)
+            self.stream.synchronize()
+            _backend.call("ticc_graph_begin", self.stream.cuda_stream)
+            try:
+                for entry in self.buckets:
+                    _set_walk(entry["bucket"].walk)
+                    self._enqueue(entry, self.scratch, self.stream)
+            finally:
+                _backend.call("ticc_graph_end", self.stream.cuda_stream, ctypes.byref(self.graph))
+        finally:
+            _set_walk((0, 0))
+
+    def _enqueue(self, entry, scratch, stream) -> int:
+        N, nb = entry["N"], entry["nb"]
+        n2 = N * N
+        base = _device.ptr(entry["W"])
+        size = ctypes.c_longlong(0 if scratch is None else scratch.numel())
+        _backend.call("ticc_ldmd_propagate_blocks_async", nb, self.n_energy, N, self.mu, self.hs.size,
+                      self.hs.ctypes.data_as(_backend._dp), _device.ptr(entry["E"]), _device.ptr(self.hs_dev), base,
+                      base + 8 * n2, base + 16 * n2, 2 * n2, (2 * self.hs.size + 1) * n2, _device.ptr(entry["Y0"]),
+                      _device.ptr(entry["Y"]), None if scratch is None else _device.ptr(scratch), ctypes.byref(size),
+                      None if stream is None else stream.cuda_stream)
+        return int(size.value)
+
+    def replay(self) -> None:
+        """Enqueue one launch of the captured graph (on this object's stream, after
+        the work already queued on the caller's current stream)."""
+        self.stream.wait_stream(torch.cuda.current_stream(self.dev))
+        _backend.call("ticc_graph_launch", self.graph, self.stream.cuda_stream)
+        torch.cuda.current_stream(self.dev).wait_stream(self.stream)
+
+    def results(self) -> list:
+        out = [None] * len(self.sizes)
+        nE = self.n_energy
+        for entry in self.buckets:
+            for k, i in enumerate(entry["bucket"].blocks):
+                n = self.sizes[i]
+                out[i] = entry["Y"][k * nE : (k + 1) * nE, :n, :n]
+        return out
+
+    def close(self) -> None:
+        if self.graph:
+            _backend.call("ticc_graph_destroy", self.graph)
+            self.graph = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
 
 
 def _timed_sweep(Np: int, batch: int, n_sector: int, walk, dev: torch.device, seed: int) -> float:

@@ -114,6 +114,56 @@ def propagate(hamiltonian, Etot, radial_sectors, config) -> torch.Tensor:
     return Y
 
 
+def block_sources(hamiltonians) -> list:
+    """For each block, the block whose contracted W it takes: blocks truncated
+    from one Hamiltonian share its W -- the largest of them is contracted and the
+    others take its leading blocks (the same numbers)."""
+    sizes = [h.basis.n_channel for h in hamiltonians]
+    source = list(range(len(hamiltonians)))
+    largest: dict = {}
+    for b, ham in enumerate(hamiltonians):
+        root = getattr(ham, "truncated_from", None)
+        if root is not None and (id(root) not in largest or sizes[b] > sizes[largest[id(root)]]):
+            largest[id(root)] = b
+    for b, ham in enumerate(hamiltonians):
+        root = getattr(ham, "truncated_from", None)
+        if root is not None:
+            source[b] = largest[id(root)]
+    return source
+
+
+def blocks_W_points(hamiltonians, pts, dev, pad: float, source, W_prev=None) -> torch.Tensor:
+    """W_base of every block at the radial points `pts`, padded to the largest
+    block with decoupled closed channels (zero coupling, diagonal `pad`): shape
+    (n_block, n_points, N_max, N_max).  W_prev = (R, W at R of every block) reuses
+    a shared first point."""
+    sizes = [h.basis.n_channel for h in hamiltonians]
+    nb, Nmax = len(hamiltonians), max(sizes)
+    Wall = torch.zeros((nb, pts.size, Nmax, Nmax), dtype=torch.float64, device=dev)
+    reuse = W_prev is not None and np.isclose(pts[0], W_prev[0], rtol=0.0, atol=1.0e-12)
+    for b, (ham, n) in enumerate(zip(hamiltonians, sizes)):
+        if source[b] != b:
+            continue
+        Wb = torch.empty((pts.size, n, n), dtype=torch.float64, device=dev)
+        if reuse:
+            Wb[0].copy_(W_prev[1][b, :n, :n])
+            ham.W_points(pts[1:], dev, out=Wb[1:])
+        else:
+            ham.W_points(pts, dev, out=Wb)
+        Wall[b, :, :n, :n] = Wb
+    for b, n in enumerate(sizes):
+        if source[b] != b:
+            Wall[b, :, :n, :n] = Wall[source[b], :, :n, :n]
+        if n < Nmax:
+            Wall[b, :, n:, n:].diagonal(dim1=-2, dim2=-1).fill_(pad)
+    return Wall
+
+
+def padding_diagonal(reduced_mass: float, energies) -> float:
+    """Diagonal of the decoupled closed channels that pad a block: far above every energy."""
+    return 2.0 * reduced_mass * float(np.max(np.abs(energies))) + 1.0e4
+
+
 def propagate_blocks(hamiltonians, Etot, radial_sectors, config) -> list:
     """Y_BF of several channel blocks swept together (runner.py:326-374): every block's
     W_base is contracted per radial window, padded to the largest block with decoupled
@@ -132,44 +182,17 @@ def propagate_blocks(hamiltonians, Etot, radial_sectors, config) -> list:
     nb, Nmax = len(hamiltonians), max(sizes)
     mu = hamiltonians[0].reduced_mass
     E = np.ascontiguousarray(energies, dtype=np.float64)
-    pad = 2.0 * mu * float(np.max(np.abs(E))) + 1.0e4  # closed, decoupled padding channels
+    pad = padding_diagonal(mu, E)
     size = window_size(len(sectors), hamiltonians[0].potential_grid_size, Nmax * int(np.ceil(np.sqrt(nb))),
                        config.memory_mb)
     hs_all = np.asarray([s.radial_half_step for s in sectors])
-    # Blocks truncated from one Hamiltonian share its W: the largest of them is
-    # contracted and the others take its leading blocks (the same numbers).
-    source = list(range(nb))
-    largest: dict = {}
-    for b, ham in enumerate(hamiltonians):
-        root = getattr(ham, "truncated_from", None)
-        if root is not None and (id(root) not in largest or sizes[b] > sizes[largest[id(root)]]):
-            largest[id(root)] = b
-    for b, ham in enumerate(hamiltonians):
-        root = getattr(ham, "truncated_from", None)
-        if root is not None:
-            source[b] = largest[id(root)]
+    source = block_sources(hamiltonians)
     Y = None
     W_prev = None
     for w0 in range(0, len(sectors), size):
         window = sectors[w0 : w0 + size]
         pts = window_points(window)
-        Wall = torch.zeros((nb, pts.size, Nmax, Nmax), dtype=torch.float64, device=dev)
-        reuse = W_prev is not None and np.isclose(pts[0], W_prev[0], rtol=0.0, atol=1.0e-12)
-        for b, (ham, n) in enumerate(zip(hamiltonians, sizes)):
-            if source[b] != b:
-                continue
-            Wb = torch.empty((pts.size, n, n), dtype=torch.float64, device=dev)
-            if reuse:
-                Wb[0].copy_(W_prev[1][b, :n, :n])
-                ham.W_points(pts[1:], dev, out=Wb[1:])
-            else:
-                ham.W_points(pts, dev, out=Wb)
-            Wall[b, :, :n, :n] = Wb
-        for b, n in enumerate(sizes):
-            if source[b] != b:
-                Wall[b, :, :n, :n] = Wall[source[b], :, :n, :n]
-            if n < Nmax:
-                Wall[b, :, n:, n:].diagonal(dim1=-2, dim2=-1).fill_(pad)
+        Wall = blocks_W_points(hamiltonians, pts, dev, pad, source, W_prev)
         if Y is None:
             Y = torch.cat([initial_logD_energies(Wall[b, 0], E, mu, dev) for b in range(nb)])
         n2 = Nmax * Nmax
