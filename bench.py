@@ -147,6 +147,8 @@ def main() -> int:
     ap.add_argument("--mixed-steps", type=int, default=1,
                     help="timed steps of the mixed-precision sweep on the same workload (0: skip its sub-record)")
     ap.add_argument("--refine-steps", type=int, default=2, help="Newton steps of the mixed-precision sweep")
+    ap.add_argument("--batching-blocks", type=int, default=256,
+                    help="blocks of the batching sub-record (captured-graph bucketed sweep; 0: skip it)")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="the timed steps' sweep (mixed: the headline line measures the precision form itself, "
                     "for its counter profiles; the default line reports it in the 'mixed' sub-record)")
@@ -230,6 +232,9 @@ def main() -> int:
     parity = None if args.max_sectors > 0 else _fixture_parity(S, w, bar)
     if parity is not None:
         _log(f"parity vs the N = 800 fixture: max|dS| {parity['max_abs_dS']:.2e} (tolerance {parity['max_tolerance']:.1e})")
+    batching = None
+    if args.batching_blocks > 0 and args.precision == "fp64":
+        batching = _bench_batching_graph(args)
     mixed = None
     if args.mixed_steps > 0 and args.precision == "fp64":
         mixed = _bench_mixed(args, step, ticc, w, nE, nS, value, S, unpack_smat)
@@ -289,6 +294,7 @@ def main() -> int:
         "max_unitarity_defect": defect,
         "parity": parity,
         "mixed": mixed,
+        "batching": batching,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -358,6 +364,82 @@ def _bench_mixed(args, step, ticc, w, nE, nS, fp64_value, S64, unpack_smat):
         "max_rel_dsigma_vs_fp64_run": dsig,
         "max_unitarity_defect": defect,
         "parity": parity,
+    }
+
+
+def _bench_batching_graph(args) -> dict:
+    """The batching form: the seeded uneven two-rotor blocks (N log-uniform in
+    40-1200, each the N lowest channels of a (J, parity) basis, J <= 8) x 32
+    energies over the first 8 sectors, bucketed by the sweep autotuner's cost model
+    and swept as one captured HIP graph (W contracted beforehand).  Against: the
+    same buckets enqueued launch by launch, and one sweep per block."""
+    import torch
+
+    import pyticc_amd as ticc
+    from pyticc_amd import synthetic
+    from pyticc_amd.propagation.buckets import (CapturedBuckets, modelled_seconds, plan_buckets, plan_singletons,
+                                                tune_sweep)
+    from pyticc_amd.scattering.hamiltonian import build_hamiltonian
+
+    t0 = time.perf_counter()
+    n_sector, n_energy, h = 8, 32, 0.009
+    specs = synthetic.rotor_rotor_block_specs(seed=3, n_block=args.batching_blocks)
+    works = synthetic.rotor_rotor_blocks(specs, n_energy=n_energy, boundaries=(4.0, 4.0 + 2 * h * n_sector),
+                                         half_step=h)
+    first = next(iter(works.values()))
+    grid = ticc.prepare_potential(first.system, first.boundaries, first.half_steps, **first.quadrature)
+    full = {k: build_hamiltonian(w.system, grid) for k, w in works.items()}
+    hams = [full[(s.Jtot, s.parity)].truncate(s.n_channel) for s in specs]
+    sizes = np.array([s.n_channel for s in specs])
+    E = first.energies
+    prop = ticc.Propagation(device="gpu", memory_mb=8192.0)
+    grid_pads = [n for n in (64, 128, 256, 384, 512, 768, 1024, 1216) if n <= max(64, sizes.max() + 31)]
+    cost = tune_sweep(grid_pads, batches=(32, 256), device="gpu")
+    plan = plan_buckets(sizes, n_energy, cost)
+    single = plan_singletons(sizes)
+    cap = CapturedBuckets(hams, E, grid.sectors, prop, plan)
+    caps = CapturedBuckets(hams, E, grid.sectors, prop, single)
+    _log(f"batching setup {time.perf_counter() - t0:.1f} s: {len(hams)} blocks in {len(plan)} buckets")
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    try:
+        t_graph = timed(cap.replay)
+        Yg = [y.clone() for y in cap.results()]
+        t_launch = timed(cap.enqueue)
+        t_single = timed(caps.replay, reps=1)
+        Ys = caps.results()
+        diff = max(float((a - b).abs().max()) for a, b in zip(Yg, Ys))
+        finite = all(bool(torch.isfinite(y).all()) for y in Yg)
+    finally:
+        cap.close()
+        caps.close()
+    units = len(hams) * n_energy * n_sector
+    credited = 4.0 * float(np.sum(sizes.astype(np.float64) ** 3)) * n_energy * n_sector
+    _log(f"batching: graph {t_graph:.3f} s, launch by launch {t_launch:.3f} s, per block {t_single:.3f} s per step")
+    return {
+        "value": units / t_graph,
+        "unit": "energy.R-steps/s",
+        "workload": f"{len(hams)} two-rotor blocks, N log-uniform 40-1200 (seed 3; the N lowest channels of a "
+        f"(J, parity) basis, J <= 8), {n_energy} energies, the first {n_sector} LDMD sectors from 4 bohr; FP64 sweep "
+        "of every block (W contracted beforehand)",
+        "n_buckets": len(plan),
+        "seconds_per_step": {"graph": t_graph, "launch_by_launch": t_launch, "per_block_graph": t_single},
+        "speedup_vs_per_block": t_single / t_graph,
+        "graph_vs_launch_by_launch": t_launch / t_graph,
+        "tflops_credited": credited / t_graph / 1e12,
+        "frac_of_fp64_peak": credited / t_graph / 1e12 / FP64_MFMA_PEAK_TFLOPS,
+        "modelled_seconds": {"bucketed": modelled_seconds(plan, n_energy, cost) * n_sector,
+                             "per_block": modelled_seconds(single, n_energy, cost) * n_sector},
+        "max_abs_diff_vs_per_block": diff,
+        "finite": finite,
     }
 
 

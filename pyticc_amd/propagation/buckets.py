@@ -304,6 +304,17 @@ class CapturedBuckets:
                       None if stream is None else stream.cuda_stream)
         return int(size.value)
 
+    def enqueue(self) -> None:
+        """The same sweeps enqueued one launch at a time, without the graph (for comparison)."""
+        self.stream.wait_stream(torch.cuda.current_stream(self.dev))
+        try:
+            for entry in self.buckets:
+                _set_walk(entry["bucket"].walk)
+                self._enqueue(entry, self.scratch, self.stream)
+        finally:
+            _set_walk((0, 0))
+        torch.cuda.current_stream(self.dev).wait_stream(self.stream)
+
     def replay(self) -> None:
         """Enqueue one launch of the captured graph (on this object's stream, after
         the work already queued on the caller's current stream)."""

@@ -1,6 +1,8 @@
 """Generate the committed golden fixtures (run on the CPU; minutes for the N=800 case).
 
     python tests/golden/make_golden.py [bessel] [cg] [atom_rotor_N66] [rotor_rotor_N800]
+    python tests/golden/make_golden.py --parts DIR [--only 4,15]   (the N = 800 energies split
+                                                                    across processes, cached in DIR)
 
 Fixtures (NumPy .npz, no pickled objects):
   bessel_mpmath.npz   Riccati-Bessel j, n, j', n' (j = sqrt(pi x/2) J_{l+1/2}(x)) and the
@@ -106,21 +108,44 @@ def make_atom_rotor_N66() -> None:
     np.savez_compressed(HERE / "smat_atom_rotor_N66.npz", **out)
 
 
-def make_rotor_rotor_N800() -> None:
+def _rotor_rotor_N800_part(w, prob, k: int, parts_dir):
+    """S of both FP64 orderings at energy index k (cached in parts_dir when given)."""
     from oracle import ldmd
-    from oracle import matching as omatch
     from oracle import pipeline as opipe
+
+    path = None if parts_dir is None else Path(parts_dir) / f"n800_energy{k}.npz"
+    if path is not None and path.exists():
+        d = np.load(path)
+        return d["S"], d["S_inv"]
+    E = w.energies[[k]]
+    Y_solve, Y_inv = opipe.propagate(prob, E, steps=(ldmd.sector_step, ldmd.sector_step_inverse))
+    S = opipe.match(prob, Y_solve, E)[2][0]
+    S_inv = opipe.match(prob, Y_inv, E)[2][0]
+    if path is not None:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        np.savez(path, S=S, S_inv=S_inv)
+    return S, S_inv
+
+
+def make_rotor_rotor_N800(parts_dir=None, only=None) -> None:
+    """The N = 800 fixture, one energy at a time; with parts_dir each energy's S
+    matrices are cached there (so separate processes can split the energies, and
+    an interrupted run resumes), and with `only` just those energy indices are
+    computed and nothing is assembled."""
+    from oracle import matching as omatch
     from pyticc_amd import synthetic
     from tests.helpers import oracle_problem
 
     w = synthetic.rotor_rotor()
     _, _, prob = oracle_problem(w)
-    E = w.energies[list(CONFIG2_ENERGIES)]
-    Y_solve, Y_inv = opipe.propagate(prob, E, steps=(ldmd.sector_step, ldmd.sector_step_inverse))
-    _, _, S = opipe.match(prob, Y_solve, E)
-    _, _, S_inv = opipe.match(prob, Y_inv, E)
+    todo = CONFIG2_ENERGIES if only is None else tuple(only)
+    parts = {k: _rotor_rotor_N800_part(w, prob, k, parts_dir) for k in todo}
+    if only is not None:
+        return
+    S = [parts[k][0] for k in CONFIG2_ENERGIES]
+    S_inv = [parts[k][1] for k in CONFIG2_ENERGIES]
     rng = np.random.default_rng(800)
-    out = {"energy_index": np.array(CONFIG2_ENERGIES), "energies": E,
+    out = {"energy_index": np.array(CONFIG2_ENERGIES), "energies": w.energies[list(CONFIG2_ENERGIES)],
            "defect": np.array([omatch.unitarity_defect(s) if s.size else 0.0 for s in S]),
            "floor": np.array([float(np.max(np.abs(a - b))) if a.size else 0.0 for a, b in zip(S, S_inv)]),
            "n_open": np.array([s.shape[0] for s in S])}
@@ -136,9 +161,24 @@ def make_rotor_rotor_N800() -> None:
             out[f"S{i}_diag"] = np.diag(s).copy()
     np.savez_compressed(HERE / "smat_rotor_rotor_N800.npz", **out)
 
-
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["bessel", "cg", "atom_rotor_N66", "rotor_rotor_N800"]
+    args = sys.argv[1:]
+    parts_dir = None
+    if "--parts" in args:  # --parts DIR [--only k1,k2]: cache (and split) the N = 800 energies
+        i = args.index("--parts")
+        parts_dir = args[i + 1]
+        del args[i : i + 2]
+    only = None
+    if "--only" in args:
+        i = args.index("--only")
+        only = [int(k) for k in args[i + 1].split(",")]
+        del args[i : i + 2]
+    if parts_dir is not None or only is not None:
+        t = time.perf_counter()
+        make_rotor_rotor_N800(parts_dir, only)
+        print(f"rotor_rotor_N800 {only or 'all'}: {time.perf_counter() - t:.1f} s", flush=True)
+        sys.exit(0)
+    which = args or ["bessel", "cg", "atom_rotor_N66", "rotor_rotor_N800"]
     for name in which:
         t = time.perf_counter()
         {"bessel": make_bessel, "cg": make_cg, "atom_rotor_N66": make_atom_rotor_N66, "rotor_rotor_N800": make_rotor_rotor_N800}[name]()

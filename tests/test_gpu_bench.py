@@ -24,7 +24,8 @@ def _bench(*args):
 
 
 def test_headline_bench_line(gpu_device):
-    d = _bench("--max-sectors", "4", "--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--profile-every", "1")
+    d = _bench("--max-sectors", "4", "--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--profile-every", "1",
+               "--batching-blocks", "6")
     for key in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better", "scaling",
                 "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline"):
         assert key in d
@@ -41,6 +42,8 @@ def test_headline_bench_line(gpu_device):
     m = d["mixed"]
     assert m["value"] > 0 and m["refine_steps"] == 2 and m["ratio_to_fp64"] > 0
     assert m["max_abs_dS_vs_fp64_run"] <= 1e-6 and m["max_unitarity_defect"] < 1e-8
+    bt = d["batching"]
+    assert bt["value"] > 0 and bt["finite"] and bt["max_abs_diff_vs_per_block"] == 0.0 and bt["n_buckets"] >= 1
 
 
 def test_batching_bench_line(gpu_device):
