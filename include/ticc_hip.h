@@ -329,6 +329,37 @@ int ticc_mlp_grid_features(int n, int n_valid, int n_grid, const double* R, int 
 int ticc_mlp_loss(int n, int n_valid, const float* Y, const float* T, void* dZ, void* dZT, float* dbias, float* loss,
                   void* stream);
 
+/*
+ * Split-BF16 forward products of the surrogate: A and B each given as BF16 high
+ * and low parts (x_lo = bf16(x - x_hi); A_lo may be NULL), C = A_hi B_hi^T +
+ * A_hi B_lo^T + A_lo B_hi^T with FP32 accumulation (about 16 significant bits).
+ * mode 1: Z = C + bias -> Cf, a = SiLU(Z) -> Cb (high), Cb_lo (low), CbT (high,
+ * transposed); mode 2: Cf = C + bias.  Shapes as ticc_mlp_gemm.
+ * split_lo: lo[i] = bf16(x[i] - bf16(x[i])) of an FP32 tensor of n elements.
+ */
+int ticc_mlp_gemm_split(int mode, int M, int N, int K, const void* A, const void* A_lo, const void* B,
+                        const void* B_lo, const float* bias, float* Cf, void* Cb, void* Cb_lo, void* CbT, void* stream);
+int ticc_mlp_split_lo(int n, const float* x, void* lo, void* stream);
+
+/*
+ * The angular-basis form of the surrogate: the network maps radial features to
+ * the coefficients u_t(R) of fixed angular functions G_t (the bispherical terms of
+ * the PES expansion), V = v_scale s(R) sum_t u_t(R) G_t.
+ * radial_features: row i of F (n x 64 BF16, FT 64 x n optional) = [x, x^2, x^3,
+ *   256 (x - bf16(x)), 256 (x^2 - bf16(x^2)), 256 (x^3 - bf16(x^3))], x = r_scale / R[i]
+ *   (R resolved to about 16 bits; padding rows past n_valid: x = 1/2).
+ * basis_loss: per row, pred = U[i][:] . G[gidx[i]][:] (ld columns, FP32), d = pred - T[i];
+ *   *loss += sum d^2 / n_valid; dZf (FP32), dZ and dZT (BF16) = 2 d G[gidx[i]][:] / n_valid
+ *   (zero rows past n_valid).  ld a multiple of 64.
+ * basis_eval: V[r][g] = scale[r] sum_{t < nt} U[r][t] GT[t][g] (U FP32 with row stride ld,
+ *   GT FP64 nt x n_grid, V FP64 nR x n_grid), nt <= 512.
+ */
+int ticc_mlp_radial_features(int n, int n_valid, const double* R, double r_scale, void* F, void* FT, void* stream);
+int ticc_mlp_basis_loss(int n, int n_valid, int ld, const float* U, const int* gidx, const float* G, const float* T,
+                        float* dZf, void* dZ, void* dZT, float* loss, void* stream);
+int ticc_mlp_basis_eval(int nR, int n_grid, int nt, int ld, const float* U, const double* GT, const double* scale,
+                        double* V, void* stream);
+
 /* out[c] += sum_r X[r][c], X (M x N), N a multiple of 64. */
 int ticc_mlp_colsum(int M, int N, const float* X, float* out, void* stream);
 

@@ -77,6 +77,11 @@ _SIGNATURES = {
     "ticc_mlp_grid_features": ([_c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _c_int, _c_d, _vp, _vp], _c_int),
     "ticc_mlp_loss": ([_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "ticc_mlp_colsum": ([_c_int, _c_int, _vp, _vp, _vp], _c_int),
+    "ticc_mlp_gemm_split": ([_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "ticc_mlp_split_lo": ([_c_int, _vp, _vp, _vp], _c_int),
+    "ticc_mlp_radial_features": ([_c_int, _c_int, _vp, _c_d, _vp, _vp, _vp], _c_int),
+    "ticc_mlp_basis_loss": ([_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "ticc_mlp_basis_eval": ([_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp], _c_int),
     "ticc_mlp_adam": (
         [_c_int, _c_int, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _c_int, _vp,
          _vp, _vp],

@@ -21,6 +21,14 @@ int mlp_grid_features(int n, int n_valid, int n_grid, const double* R, int n_coo
 int mlp_loss(int n, int n_valid, const float* Y, const float* T, void* dZ, void* dZT, float* dbias, float* loss,
              hipStream_t stream);
 int mlp_colsum(int M, int N, const float* X, float* out, hipStream_t stream);
+int mlp_gemm_split(int mode, int M, int N, int K, const void* A, const void* A_lo, const void* B, const void* B_lo,
+                   const float* bias, float* Cf, void* Cb, void* Cb_lo, void* CbT, hipStream_t stream);
+int mlp_split_lo(int n, const float* x, void* lo, hipStream_t stream);
+int mlp_radial_features(int n, int n_valid, const double* R, double r_scale, void* F, void* FT, hipStream_t stream);
+int mlp_basis_loss(int n, int n_valid, int ld, const float* U, const int* gidx, const float* G, const float* T,
+                   float* dZf, void* dZ, void* dZT, float* loss, hipStream_t stream);
+int mlp_basis_eval(int nR, int n_grid, int nt, int ld, const float* U, const double* GT, const double* scale, double* V,
+                   hipStream_t stream);
 int mlp_adam(int rows, int cols, float* theta, const float* grad, float* m, float* v, float lr, float b1, float b2,
              float eps, int step, void* bf, void* bfT, hipStream_t stream);
 int mlp_to_bf16(int rows, int cols, const float* x, void* bf, void* bfT, hipStream_t stream);
@@ -416,6 +424,39 @@ int ticc_mlp_loss(int n, int n_valid, const float* Y, const float* T, void* dZ, 
   if (n <= 0 || n_valid <= 0 || n_valid > n || !Y || !T || !dZ || !dZT || !dbias || !loss)
     return fail(TICC_ERR_VALUE, "mlp_loss: invalid arguments");
   return ticc::mlp_loss(n, n_valid, Y, T, dZ, dZT, dbias, loss, as_stream(stream));
+}
+
+int ticc_mlp_gemm_split(int mode, int M, int N, int K, const void* A, const void* A_lo, const void* B,
+                        const void* B_lo, const float* bias, float* Cf, void* Cb, void* Cb_lo, void* CbT, void* stream) {
+  if (!A || !B || !B_lo || !bias || !Cf) return fail(TICC_ERR_VALUE, "mlp_gemm_split: null operand");
+  if (mode == 1 && (!Cb || !Cb_lo || !CbT)) return fail(TICC_ERR_VALUE, "mlp_gemm_split: null activation output");
+  return ticc::mlp_gemm_split(mode, M, N, K, A, A_lo, B, B_lo, bias, Cf, Cb, Cb_lo, CbT, as_stream(stream));
+}
+
+int ticc_mlp_split_lo(int n, const float* x, void* lo, void* stream) {
+  if (n <= 0 || !x || !lo) return fail(TICC_ERR_VALUE, "mlp_split_lo: invalid arguments");
+  return ticc::mlp_split_lo(n, x, lo, as_stream(stream));
+}
+
+int ticc_mlp_radial_features(int n, int n_valid, const double* R, double r_scale, void* F, void* FT, void* stream) {
+  if (n <= 0 || n_valid < 0 || n_valid > n || !F || (n_valid > 0 && !R))
+    return fail(TICC_ERR_VALUE, "mlp_radial_features: invalid arguments");
+  return ticc::mlp_radial_features(n, n_valid, R, r_scale, F, FT, as_stream(stream));
+}
+
+int ticc_mlp_basis_loss(int n, int n_valid, int ld, const float* U, const int* gidx, const float* G, const float* T,
+                        float* dZf, void* dZ, void* dZT, float* loss, void* stream) {
+  if (n <= 0 || n_valid <= 0 || n_valid > n || ld <= 0 || ld % 64 || !U || !gidx || !G || !T || !dZf || !dZ || !dZT ||
+      !loss)
+    return fail(TICC_ERR_VALUE, "mlp_basis_loss: invalid arguments");
+  return ticc::mlp_basis_loss(n, n_valid, ld, U, gidx, G, T, dZf, dZ, dZT, loss, as_stream(stream));
+}
+
+int ticc_mlp_basis_eval(int nR, int n_grid, int nt, int ld, const float* U, const double* GT, const double* scale,
+                        double* V, void* stream) {
+  if (nR <= 0 || n_grid <= 0 || nt <= 0 || nt > 512 || nt > ld || !U || !GT || !scale || !V)
+    return fail(TICC_ERR_VALUE, "mlp_basis_eval: invalid arguments");
+  return ticc::mlp_basis_eval(nR, n_grid, nt, ld, U, GT, scale, V, as_stream(stream));
 }
 
 int ticc_mlp_colsum(int M, int N, const float* X, float* out, void* stream) {

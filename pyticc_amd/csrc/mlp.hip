@@ -45,11 +45,21 @@ enum { EPI_STORE = 0, EPI_HIDDEN = 1, EPI_OUTPUT = 2, EPI_BACK = 3 };
 //   EPI_HIDDEN  Z = C + bias -> Cf; a = SiLU(Z) -> Cb (bf16) and CbT (bf16, N x M)
 //   EPI_OUTPUT  Cf = C + bias                           (network output)
 //   EPI_BACK    dZ = C * SiLU'(Zin) -> Cf, Cb and CbT    (back through a hidden layer)
-template <int EPI>
+//
+// Split form (SPLIT): each operand is the sum of a BF16 high part and a BF16 low
+// part (x_lo = bf16(x - x_hi)), and C = A_hi B_hi + A_hi B_lo + A_lo B_hi (three
+// MFMAs per step, FP32 accumulation): products to about 16 significant bits
+// instead of 8.  A_lo may be null (an input without a low part).  EPI_HIDDEN then
+// also writes the activation's low part Cb_lo.
+
+template <int EPI, bool SPLIT = false>
 __global__ __launch_bounds__(256) void k_mlp_gemm(int M, int N, int K, const uint16_t* __restrict__ A,
                                                   const uint16_t* __restrict__ B, const float* __restrict__ bias,
                                                   const float* __restrict__ Zin, float* __restrict__ Cf,
-                                                  uint16_t* __restrict__ Cb, uint16_t* __restrict__ CbT) {
+                                                  uint16_t* __restrict__ Cb, uint16_t* __restrict__ CbT,
+                                                  const uint16_t* __restrict__ A_lo = nullptr,
+                                                  const uint16_t* __restrict__ B_lo = nullptr,
+                                                  uint16_t* __restrict__ Cb_lo = nullptr) {
   const int tn = N / 64;
   const int m0 = (blockIdx.x / tn) * 64, n0 = (blockIdx.x % tn) * 64;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
@@ -71,6 +81,21 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(int M, int N, int K, const uin
 #pragma unroll
       for (int bj = 0; bj < 2; ++bj)
         acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
+    if constexpr (SPLIT) {
+      bf16x4 al[2], bl[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        bl[t] = *reinterpret_cast<const bf16x4*>(B_lo + (size_t)(wn + 16 * t + r) * K + k0 + 4 * q);
+        if (A_lo) al[t] = *reinterpret_cast<const bf16x4*>(A_lo + (size_t)(wm + 16 * t + r) * K + k0 + 4 * q);
+      }
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi], bl[bj], acc[bi][bj], 0, 0, 0);
+          if (A_lo) acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[bi], b[bj], acc[bi][bj], 0, 0, 0);
+        }
+    }
   }
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
@@ -89,9 +114,11 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(int M, int N, int K, const uin
         } else if (EPI == EPI_HIDDEN) {
           const float z = v + bias[col];
           Cf[o] = z;
-          const uint16_t h = to_bf16(z * sigmoid(z));
+          const float act = z * sigmoid(z);
+          const uint16_t h = to_bf16(act);
           Cb[o] = h;
           CbT[(size_t)col * M + row] = h;
+          if constexpr (SPLIT) Cb_lo[o] = to_bf16(act - __uint_as_float((uint32_t)h << 16));
         } else {
           const float z = Zin[o];
           const float s = sigmoid(z);
@@ -229,6 +256,86 @@ __global__ void k_mlp_to_bf16(int rows, int cols, const float* __restrict__ x, u
   }
 }
 
+// Radial features of the angular-basis network, x = r_scale / R_i: columns 0-2
+// hold x, x^2, x^3 and columns 3-5 the parts BF16 rounding drops from them,
+// 256 (x^k - bf16(x^k)), so the network sees R to about 16 bits instead of the 8 of
+// one BF16 input (zero beyond column 5; padding rows use x = 1/2).
+__device__ __forceinline__ float bf16_value(float v) { return __uint_as_float((uint32_t)to_bf16(v) << 16); }
+
+// lo[i] = bf16(x[i] - bf16(x[i])): the low BF16 part of an FP32 tensor.
+__global__ void k_mlp_split_lo(int n, const float* __restrict__ x, uint16_t* __restrict__ lo) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    lo[i] = to_bf16(x[i] - bf16_value(x[i]));
+}
+
+__global__ void k_mlp_radial_features(int n, int n_valid, const double* __restrict__ R, double r_scale,
+                                      uint16_t* __restrict__ F, uint16_t* __restrict__ FT) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double x = i < n_valid ? r_scale / R[i] : 0.5;
+    const double pw[3] = {x, x * x, x * x * x};
+    for (int c = 0; c < 64; ++c) {
+      float v = 0.0f;
+      if (c < 3) {
+        v = (float)pw[c];
+      } else if (c < 6) {
+        const float p = (float)pw[c - 3];
+        v = 256.0f * (p - bf16_value(p));
+      }
+      const uint16_t h = to_bf16(v);
+      F[(size_t)i * 64 + c] = h;
+      if (FT) FT[(size_t)c * n + i] = h;
+    }
+  }
+}
+
+// Loss of the angular-basis head.  Row i's prediction is the dot product of the
+// network output U[i][:] (ld columns, FP32) with the angular functions of its
+// sample, G[gidx[i]][:] (FP32, ld columns, zero past the basis): d = pred - T[i],
+// loss += sum d^2 / n_valid, and the output gradient dZ[i][t] = 2 d G[gidx[i]][t] /
+// n_valid -> dZf (FP32, for the bias gradient), dZ and dZT (BF16).  Rows past
+// n_valid get zero gradients.  One wave per row.
+__global__ __launch_bounds__(256) void k_mlp_basis_loss(int n, int n_valid, int ld, const float* __restrict__ U,
+                                                        const int* __restrict__ gidx, const float* __restrict__ G,
+                                                        const float* __restrict__ T, float* __restrict__ dZf,
+                                                        uint16_t* __restrict__ dZ, uint16_t* __restrict__ dZT,
+                                                        float* __restrict__ loss) {
+  const int lane = threadIdx.x & 63;
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (row >= n) return;
+  const bool live = row < n_valid;
+  const float* g = G + (size_t)(live ? gidx[row] : 0) * ld;
+  float part = 0.0f;
+  for (int t = lane; t < ld; t += 64) part += U[(size_t)row * ld + t] * g[t];
+  for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, 64);
+  const float d = live ? part - T[row] : 0.0f;
+  const float c = 2.0f * d / (float)n_valid;
+  for (int t = lane; t < ld; t += 64) {
+    const float v = c * g[t];
+    dZf[(size_t)row * ld + t] = v;
+    const uint16_t h = to_bf16(v);
+    dZ[(size_t)row * ld + t] = h;
+    dZT[(size_t)t * n + row] = h;
+  }
+  if (lane == 0 && live) atomicAdd(loss, d * d / (float)n_valid);
+}
+
+// V[r][g] = scale[r] * sum_t U[r][t] GT[t][g] for the first nt of ld network
+// outputs (U FP32, GT = the angular functions transposed, FP64): the network's
+// PES on a grid of n_grid angular points at nR radial points.
+__global__ __launch_bounds__(256) void k_mlp_basis_eval(int nR, int n_grid, int nt, int ld, const float* __restrict__ U,
+                                                        const double* __restrict__ GT, const double* __restrict__ scale,
+                                                        double* __restrict__ V) {
+  __shared__ double u[512];
+  const int r = blockIdx.y;
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) u[t] = (double)U[(size_t)r * ld + t];
+  __syncthreads();
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_grid) return;
+  double acc = 0.0;
+  for (int t = 0; t < nt; ++t) acc = fma(u[t], GT[(size_t)t * n_grid + g], acc);
+  V[(size_t)r * n_grid + g] = scale[r] * acc;
+}
+
 int blocks_for(long long n) {
   long long b = (n + 255) / 256;
   return (int)(b > 4096 ? 4096 : (b < 1 ? 1 : b));
@@ -265,6 +372,34 @@ int mlp_gemm(int mode, int M, int N, int K, const void* A, const void* B, const 
   return TICC_OK;
 }
 
+int mlp_gemm_split(int mode, int M, int N, int K, const void* A, const void* A_lo, const void* B, const void* B_lo,
+                   const float* bias, float* Cf, void* Cb, void* Cb_lo, void* CbT, hipStream_t stream) {
+  if (M <= 0 || N <= 0 || K <= 0 || M % 64 || N % 64 || K % 16)
+    return fail(TICC_ERR_VALUE, "mlp_gemm_split: M, N must be positive multiples of 64 and K of 16 (got %d, %d, %d)", M,
+                N, K);
+  const auto* a = static_cast<const uint16_t*>(A);
+  const auto* al = static_cast<const uint16_t*>(A_lo);
+  const auto* b = static_cast<const uint16_t*>(B);
+  const auto* bl = static_cast<const uint16_t*>(B_lo);
+  const dim3 grid((M / 64) * (N / 64));
+  if (mode == EPI_HIDDEN)
+    hipLaunchKernelGGL((k_mlp_gemm<EPI_HIDDEN, true>), grid, dim3(256), 0, stream, M, N, K, a, b, bias, nullptr, Cf,
+                       static_cast<uint16_t*>(Cb), static_cast<uint16_t*>(CbT), al, bl, static_cast<uint16_t*>(Cb_lo));
+  else if (mode == EPI_OUTPUT)
+    hipLaunchKernelGGL((k_mlp_gemm<EPI_OUTPUT, true>), grid, dim3(256), 0, stream, M, N, K, a, b, bias, nullptr, Cf,
+                       nullptr, nullptr, al, bl, nullptr);
+  else
+    return fail(TICC_ERR_VALUE, "mlp_gemm_split: mode must be 1 (hidden) or 2 (output), got %d", mode);
+  TICC_CHECK_LAUNCH();
+  return TICC_OK;
+}
+
+int mlp_split_lo(int n, const float* x, void* lo, hipStream_t stream) {
+  hipLaunchKernelGGL(k_mlp_split_lo, dim3(blocks_for(n)), dim3(256), 0, stream, n, x, static_cast<uint16_t*>(lo));
+  TICC_CHECK_LAUNCH();
+  return TICC_OK;
+}
+
 int mlp_features(int n, int n_valid, const double* X4, double r_scale, void* F, void* FT, hipStream_t stream) {
   hipLaunchKernelGGL(k_mlp_features, dim3(blocks_for(n)), dim3(256), 0, stream, n, n_valid, X4, r_scale,
                      static_cast<uint16_t*>(F), static_cast<uint16_t*>(FT));
@@ -284,6 +419,29 @@ int mlp_loss(int n, int n_valid, const float* Y, const float* T, void* dZ, void*
              hipStream_t stream) {
   hipLaunchKernelGGL(k_mlp_loss, dim3(blocks_for(n) < 1024 ? blocks_for(n) : 1024), dim3(256), 0, stream, n, n_valid,
                      Y, T, static_cast<uint16_t*>(dZ), static_cast<uint16_t*>(dZT), dbias, loss);
+  TICC_CHECK_LAUNCH();
+  return TICC_OK;
+}
+
+int mlp_radial_features(int n, int n_valid, const double* R, double r_scale, void* F, void* FT, hipStream_t stream) {
+  hipLaunchKernelGGL(k_mlp_radial_features, dim3(blocks_for(n)), dim3(256), 0, stream, n, n_valid, R, r_scale,
+                     static_cast<uint16_t*>(F), static_cast<uint16_t*>(FT));
+  TICC_CHECK_LAUNCH();
+  return TICC_OK;
+}
+
+int mlp_basis_loss(int n, int n_valid, int ld, const float* U, const int* gidx, const float* G, const float* T,
+                   float* dZf, void* dZ, void* dZT, float* loss, hipStream_t stream) {
+  hipLaunchKernelGGL(k_mlp_basis_loss, dim3((n + 3) / 4), dim3(256), 0, stream, n, n_valid, ld, U, gidx, G, T, dZf,
+                     static_cast<uint16_t*>(dZ), static_cast<uint16_t*>(dZT), loss);
+  TICC_CHECK_LAUNCH();
+  return TICC_OK;
+}
+
+int mlp_basis_eval(int nR, int n_grid, int nt, int ld, const float* U, const double* GT, const double* scale, double* V,
+                   hipStream_t stream) {
+  hipLaunchKernelGGL(k_mlp_basis_eval, dim3((n_grid + 255) / 256, nR), dim3(256), 0, stream, nR, n_grid, nt, ld, U, GT,
+                     scale, V);
   TICC_CHECK_LAUNCH();
   return TICC_OK;
 }
