@@ -11,6 +11,18 @@
 // writes both mirror images, so W is exactly symmetric.  Splitting the
 // angular sum keeps a small block's few tiles from running as a handful of
 // long serial waves.
+//
+// With enough tiles x radial points to fill the chip, k_contract_lds instead
+// gives each 4-wave workgroup one tile and 4 PW radial points: the tile's two
+// row sets (32 rows x KC angular points each) and the points' V values are
+// staged in LDS with coalesced row loads, once per angular chunk for all the
+// points, and each wave runs the MFMAs of its PW points from there.  Per
+// point the MFMA sequence (angular steps of 4 in order, the real rows and then
+// the imaginary rows into one accumulator) is that of k_contract<1>, so the
+// two give the same W bit for bit.
+#include <cstdlib>
+#include <cstring>
+
 #include "ticc_common.h"
 
 namespace ticc {
@@ -133,6 +145,125 @@ __global__ __launch_bounds__(64 * CW) void k_contract(const int4* __restrict__ t
       }
 }
 
+// The contraction's form, chosen by size (0) or forced for comparisons by
+// TICC_CONTRACT_FORM: "wave" (k_contract<1>), "lds1", "lds4" (k_contract_lds<1|4>)
+// or "split" (k_contract<4>).
+enum { CF_AUTO, CF_WAVE, CF_LDS1, CF_LDS4, CF_SPLIT };
+int contract_form() {
+  const char* env = getenv("TICC_CONTRACT_FORM");
+  if (!env) return CF_AUTO;
+  if (!strcmp(env, "wave")) return CF_WAVE;
+  if (!strcmp(env, "lds1")) return CF_LDS1;
+  if (!strcmp(env, "lds4")) return CF_LDS4;
+  if (!strcmp(env, "split")) return CF_SPLIT;
+  return CF_AUTO;
+}
+
+constexpr int CKC = 32;  // angular points per LDS chunk
+
+template <int PW>
+__global__ __launch_bounds__(256, 2) void k_contract_lds(const int4* __restrict__ tiles, const int* __restrict__ chan,
+                                                      const double* __restrict__ Br, const double* __restrict__ Bi,
+                                                      int n_ang, double norm, int nR, const double* __restrict__ R,
+                                                      const double* __restrict__ V, const double* __restrict__ U,
+                                                      const double* __restrict__ Eint, double two_mu, int N,
+                                                      double* __restrict__ W) {
+  constexpr int P = 4 * PW;  // radial points per workgroup
+  __shared__ double As[32][CKC + 1];
+  __shared__ double Bs[32][CKC + 1];
+  __shared__ double Vs[P][CKC];
+  const int4 tl = tiles[blockIdx.x];
+  const int o = tl.x, nk = tl.y;
+  const int a0 = 32 * tl.z, b0 = 32 * tl.w;
+  const int r0 = blockIdx.y * P;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int lr = lane & 15;
+  const int lq = lane >> 4;
+  // staging: each thread loads column (t & 31) of rows (t >> 5) + 8 i
+  const int sc = t & 31;
+  const int sr = t >> 5;
+
+  double4_t acc[PW][2][2];
+#pragma unroll
+  for (int p = 0; p < PW; ++p)
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) acc[p][bi][bj] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+  for (int part = 0; part < 2; ++part) {
+    const double* Bm = (part == 0) ? Br : Bi;
+    if (Bm == nullptr) break;
+    for (int k0 = 0; k0 < n_ang; k0 += CKC) {
+      const int g = k0 + sc;
+      const bool vg = g < n_ang;
+      __syncthreads();  // the previous chunk has been consumed
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = sr + 8 * i;
+        const int la = a0 + row, lb = b0 + row;
+        As[row][sc] = (vg && la < nk) ? Bm[(size_t)(o + la) * n_ang + g] : 0.0;
+        Bs[row][sc] = (vg && lb < nk) ? Bm[(size_t)(o + lb) * n_ang + g] : 0.0;
+      }
+      for (int idx = t; idx < P * CKC; idx += 256) {
+        const int p = idx / CKC, c = idx % CKC;
+        const int r = r0 + p;
+        Vs[p][c] = (r < nR && k0 + c < n_ang) ? V[(size_t)r * n_ang + k0 + c] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll 2
+      for (int ks = 0; ks < CKC / 4; ++ks) {
+        const int c = 4 * ks + lq;
+        double a[2], b[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          a[q] = As[16 * q + lr][c];
+          b[q] = Bs[16 * q + lr][c];
+        }
+#pragma unroll
+        for (int p = 0; p < PW; ++p) {
+          const double vv = Vs[wave * PW + p][c];
+#pragma unroll
+          for (int bi = 0; bi < 2; ++bi) {
+            const double av = a[bi] * vv;
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+              acc[p][bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[bj], acc[p][bi][bj], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int p = 0; p < PW; ++p) {
+    const int r = r0 + wave * PW + p;
+    if (r >= nR) continue;
+    const double RR = R[r];
+    const double R2 = RR * RR;
+    double* Wr = W + (size_t)r * N * N;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int la = a0 + 16 * bi + mfma_drow(lq, q);
+          const int lb = b0 + 16 * bj + lr;
+          if (la >= nk || lb >= nk) continue;
+          if (a0 == b0 && la > lb) continue;
+          const int ca = chan[o + la];
+          const int cb = chan[o + lb];
+          double w = U[(size_t)ca * N + cb] / R2 + two_mu * (norm * acc[p][bi][bj][q]);
+          if (ca == cb) w += two_mu * Eint[ca];
+          Wr[(size_t)ca * N + cb] = w;
+          Wr[(size_t)cb * N + ca] = w;
+        }
+  }
+}
+
 }  // namespace
 
 int contract_w(int N, int n_ang, int n_block, const int* block_offsets, const int* block_channels,
@@ -178,10 +309,24 @@ int contract_w(int N, int n_ang, int n_block, const int* block_offsets, const in
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   delete[] host_tiles;
   if (e != hipSuccess) return fail(TICC_ERR_HIP, "tile upload failed: %s", hipGetErrorString(e));
-  // Many tiles fill the chip with one wave each; few (a small block's) split
-  // each tile's angular sum over four waves instead.
-  const long long waves = (long long)n_tiles * nR;
-  if (waves >= 4096)
+  // Enough (tile, radial point group) workgroups to fill the chip: the LDS-staged
+  // form, 16 points per workgroup, else 4; only a handful of tiles and points
+  // (a small block's few radial points) split each tile's angular sum over four
+  // waves instead.
+  const long long groups16 = (long long)n_tiles * ((nR + 15) / 16);
+  const long long groups4 = (long long)n_tiles * ((nR + 3) / 4);
+  int form = contract_form();
+  if (form == CF_AUTO)
+    form = groups16 >= 512 ? CF_LDS4 : groups4 >= 256 ? CF_LDS1 : (long long)n_tiles * nR >= 4096 ? CF_WAVE : CF_SPLIT;
+  if (form == CF_LDS4)
+    hipLaunchKernelGGL(k_contract_lds<4>, dim3(n_tiles, (nR + 15) / 16), dim3(256), 0, stream,
+                       static_cast<const int4*>(ws), block_channels, Br, Bi, n_ang, norm, nR, R, V, U, Eint,
+                       2.0 * mu, N, W);
+  else if (form == CF_LDS1)
+    hipLaunchKernelGGL(k_contract_lds<1>, dim3(n_tiles, (nR + 3) / 4), dim3(256), 0, stream,
+                       static_cast<const int4*>(ws), block_channels, Br, Bi, n_ang, norm, nR, R, V, U, Eint,
+                       2.0 * mu, N, W);
+  else if (form == CF_WAVE)
     hipLaunchKernelGGL(k_contract<1>, dim3(n_tiles, nR), dim3(64), 0, stream, static_cast<const int4*>(ws),
                        block_channels, Br, Bi, n_ang, norm, R, V, U, Eint, 2.0 * mu, N, W);
   else

@@ -148,3 +148,39 @@ def test_contraction_diatom_diatom_matches_oracle_and_identity(gpu_device):
     blocks = [(np.asarray(vb.channel_indices[K]), vb.B_real[K], vb.B_imag[K]) for K in sorted(vb.channel_indices)]
     ref = ocontract.contract(blocks, N, vb.normalization, V)
     np.testing.assert_allclose(_device.to_host(W), ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+
+
+def test_contraction_forms_agree(gpu_device, monkeypatch):
+    """The LDS-staged contraction (16 or 4 radial points per workgroup) runs the same
+    MFMA sequence per point as one wave per tile and point: W bit for bit equal; the
+    form splitting each angular sum over four waves agrees to rounding.  Radial
+    points and angular points both leave ragged tails (37 points, 1008 angles)."""
+    from pyticc_amd import build_ScattSystem, rigid_rotor
+    from pyticc_amd.basis.angle import gauss_legendre_dvr
+
+    sysm = build_ScattSystem(rigid_rotor(1e-4, 3), rigid_rotor(1e-4, 4), Jtot=3, system_parity=-1)
+    ctX, wX = gauss_legendre_dvr(-1, 1, 6)
+    ctY, wY = gauss_legendre_dvr(-1, 1, 7)
+    phi, wphi = gauss_legendre_dvr(0, np.pi, 24)
+    vb = prepare_diatom_diatom(sysm.basis, sysm.monomer_X.rovib, sysm.monomer_Y.rovib, ctX, wX, ctY, wY, phi, wphi)
+    N = sysm.basis.n_channel
+    n_ang = vb.B_real[0].shape[1]
+    rng = np.random.default_rng(9)
+    R = np.linspace(3.0, 9.0, 37)
+    V = rng.standard_normal((R.size, n_ang))
+    U = rng.standard_normal((N, N))
+    U = U + U.T
+    Ei = rng.uniform(0, 1e-3, N)
+    dV, dU, dE = (_device.to_device(x, gpu_device) for x in (V, U, Ei))
+    got = {}
+    for form in ("wave", "lds1", "lds4", "split"):
+        monkeypatch.setenv("TICC_CONTRACT_FORM", form)
+        print(f"contraction form {form}, N={N}, n_ang={n_ang}, {R.size} points", flush=True)
+        got[form] = _device.to_host(contract_w_device(vb, R, dV, dU, dE, 77.0, gpu_device))
+    assert N > 100 and n_ang % 32 != 0
+    np.testing.assert_array_equal(got["lds1"], got["wave"])
+    np.testing.assert_array_equal(got["lds4"], got["wave"])
+    blocks = [(np.asarray(vb.channel_indices[K]), vb.B_real[K], vb.B_imag[K]) for K in sorted(vb.channel_indices)]
+    ref = ocontract.w_base(R, 77.0, Ei, U, ocontract.contract(blocks, N, vb.normalization, V))
+    for form in ("wave", "split"):
+        np.testing.assert_allclose(got[form], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
