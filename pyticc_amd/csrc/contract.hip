@@ -114,7 +114,7 @@ __global__ __launch_bounds__(64 * CW) void k_contract(const int4* __restrict__ t
   }
   if (CW > 1) __syncthreads();
   if (wave > 0) return;
-#pragma unroll
+#pragma unroll 1
   for (int w = 0; w < CW - 1; ++w)
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
@@ -146,9 +146,9 @@ __global__ __launch_bounds__(64 * CW) void k_contract(const int4* __restrict__ t
 }
 
 // The contraction's form, chosen by size (0) or forced for comparisons by
-// TICC_CONTRACT_FORM: "wave" (k_contract<1>), "lds1", "lds4" (k_contract_lds<1|4>)
-// or "split" (k_contract<4>).
-enum { CF_AUTO, CF_WAVE, CF_LDS1, CF_LDS4, CF_SPLIT };
+// TICC_CONTRACT_FORM: "wave" (k_contract<1>), "lds1", "lds4" (k_contract_lds<1|4>),
+// "split", "split8" or "split16" (k_contract<4|8|16>).
+enum { CF_AUTO, CF_WAVE, CF_LDS1, CF_LDS4, CF_SPLIT, CF_SPLIT8, CF_SPLIT16 };
 int contract_form() {
   const char* env = getenv("TICC_CONTRACT_FORM");
   if (!env) return CF_AUTO;
@@ -156,6 +156,8 @@ int contract_form() {
   if (!strcmp(env, "lds1")) return CF_LDS1;
   if (!strcmp(env, "lds4")) return CF_LDS4;
   if (!strcmp(env, "split")) return CF_SPLIT;
+  if (!strcmp(env, "split8")) return CF_SPLIT8;
+  if (!strcmp(env, "split16")) return CF_SPLIT16;
   return CF_AUTO;
 }
 
@@ -309,15 +311,17 @@ int contract_w(int N, int n_ang, int n_block, const int* block_offsets, const in
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   delete[] host_tiles;
   if (e != hipSuccess) return fail(TICC_ERR_HIP, "tile upload failed: %s", hipGetErrorString(e));
-  // Enough (tile, radial point group) workgroups to fill the chip: the LDS-staged
-  // form, 16 points per workgroup, else 4; only a handful of tiles and points
-  // (a small block's few radial points) split each tile's angular sum over four
-  // waves instead.
-  const long long groups16 = (long long)n_tiles * ((nR + 15) / 16);
-  const long long groups4 = (long long)n_tiles * ((nR + 3) / 4);
+  // The form depends on the number of radial points only, so that every block
+  // contracted over the same window rounds its W the same way (a bucket of blocks
+  // truncated from one basis takes the largest member's W for all of them, bit for
+  // bit what each would contract alone).  Windows of 32 points or more: the
+  // LDS-staged form, 16 points per workgroup (N = 800, 256 points: 5.5 ms against
+  // 12.7 ms for one wave per tile and point).  Fewer points (short sweeps, a last
+  // short window) leave too few workgroups for that, and each tile's angular sum is
+  // split over 16 waves instead (17 points: 0.14-0.32 ms for 4-27 tiles, 1.5-2.2x
+  // faster than four waves).
   int form = contract_form();
-  if (form == CF_AUTO)
-    form = groups16 >= 512 ? CF_LDS4 : groups4 >= 256 ? CF_LDS1 : (long long)n_tiles * nR >= 4096 ? CF_WAVE : CF_SPLIT;
+  if (form == CF_AUTO) form = nR >= 32 ? CF_LDS4 : CF_SPLIT16;
   if (form == CF_LDS4)
     hipLaunchKernelGGL(k_contract_lds<4>, dim3(n_tiles, (nR + 15) / 16), dim3(256), 0, stream,
                        static_cast<const int4*>(ws), block_channels, Br, Bi, n_ang, norm, nR, R, V, U, Eint,
@@ -328,6 +332,12 @@ int contract_w(int N, int n_ang, int n_block, const int* block_offsets, const in
                        2.0 * mu, N, W);
   else if (form == CF_WAVE)
     hipLaunchKernelGGL(k_contract<1>, dim3(n_tiles, nR), dim3(64), 0, stream, static_cast<const int4*>(ws),
+                       block_channels, Br, Bi, n_ang, norm, R, V, U, Eint, 2.0 * mu, N, W);
+  else if (form == CF_SPLIT8)
+    hipLaunchKernelGGL(k_contract<8>, dim3(n_tiles, nR), dim3(512), 0, stream, static_cast<const int4*>(ws),
+                       block_channels, Br, Bi, n_ang, norm, R, V, U, Eint, 2.0 * mu, N, W);
+  else if (form == CF_SPLIT16)
+    hipLaunchKernelGGL(k_contract<16>, dim3(n_tiles, nR), dim3(1024), 0, stream, static_cast<const int4*>(ws),
                        block_channels, Br, Bi, n_ang, norm, R, V, U, Eint, 2.0 * mu, N, W);
   else
     hipLaunchKernelGGL(k_contract<4>, dim3(n_tiles, nR), dim3(256), 0, stream, static_cast<const int4*>(ws),

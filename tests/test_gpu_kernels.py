@@ -153,7 +153,7 @@ def test_contraction_diatom_diatom_matches_oracle_and_identity(gpu_device):
 def test_contraction_forms_agree(gpu_device, monkeypatch):
     """The LDS-staged contraction (16 or 4 radial points per workgroup) runs the same
     MFMA sequence per point as one wave per tile and point: W bit for bit equal; the
-    form splitting each angular sum over four waves agrees to rounding.  Radial
+    forms splitting each angular sum over 4, 8 or 16 waves agree to rounding.  Radial
     points and angular points both leave ragged tails (37 points, 1008 angles)."""
     from pyticc_amd import build_ScattSystem, rigid_rotor
     from pyticc_amd.basis.angle import gauss_legendre_dvr
@@ -173,7 +173,7 @@ def test_contraction_forms_agree(gpu_device, monkeypatch):
     Ei = rng.uniform(0, 1e-3, N)
     dV, dU, dE = (_device.to_device(x, gpu_device) for x in (V, U, Ei))
     got = {}
-    for form in ("wave", "lds1", "lds4", "split"):
+    for form in ("wave", "lds1", "lds4", "split", "split8", "split16"):
         monkeypatch.setenv("TICC_CONTRACT_FORM", form)
         print(f"contraction form {form}, N={N}, n_ang={n_ang}, {R.size} points", flush=True)
         got[form] = _device.to_host(contract_w_device(vb, R, dV, dU, dE, 77.0, gpu_device))
@@ -182,5 +182,5 @@ def test_contraction_forms_agree(gpu_device, monkeypatch):
     np.testing.assert_array_equal(got["lds4"], got["wave"])
     blocks = [(np.asarray(vb.channel_indices[K]), vb.B_real[K], vb.B_imag[K]) for K in sorted(vb.channel_indices)]
     ref = ocontract.w_base(R, 77.0, Ei, U, ocontract.contract(blocks, N, vb.normalization, V))
-    for form in ("wave", "split"):
+    for form in ("wave", "split", "split8", "split16"):
         np.testing.assert_allclose(got[form], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
