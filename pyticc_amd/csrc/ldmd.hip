@@ -86,29 +86,37 @@ __global__ void k_form1(const double* __restrict__ Y, const double* __restrict__
 }
 
 // A2 = (Ym + Q_m) + diag y1,  Ym = (Q_m + diag y1) - y2 (X1 y2).
-__global__ void k_form2(const double* __restrict__ X1, const int* __restrict__ pinv, const double* __restrict__ Qm0,
-                        const double* __restrict__ y1, const double* __restrict__ y2, int N, int Np,
-                        double* __restrict__ A, int item0, int per, long long qstride) {
+// One wave per row (4 rows per 256-thread workgroup), lanes along the row.
+__global__ __launch_bounds__(256) void k_form2(const double* __restrict__ X1, const int* __restrict__ pinv,
+                                               const double* __restrict__ Qm0, const double* __restrict__ y1,
+                                               const double* __restrict__ y2, int N, int Np, double* __restrict__ A,
+                                               int item0, int per, long long qstride) {
   const int e = blockIdx.y;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= Np) return;
+  const int lane = threadIdx.x & 63;
   const size_t n2 = (size_t)Np * Np;
-  const double* Qm = Qm0 + block_off(item0, e, per, qstride);
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n2; idx += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(idx / Np);
-    const int j = (int)(idx % Np);
+  const double* Qr = Qm0 + block_off(item0, e, per, qstride) + (size_t)i * Np;
+  const double* Xr = X1 + e * n2 + (size_t)i * Np;
+  const int* pe = pinv + (size_t)e * Np;
+  const double* y2e = y2 + (size_t)e * Np;
+  double* Ar = A + e * n2 + (size_t)i * Np;
+  const double y1i = (i < N) ? y1[(size_t)e * Np + i] : 0.0;
+  const double y2i = (i < N) ? y2e[i] : 0.0;
+  for (int j = lane; j < Np; j += 64) {
     double v;
     if (i < N && j < N) {
-      const double x = X1[e * n2 + (size_t)i * Np + pinv[(size_t)e * Np + j]];
-      const double q = Qm[idx];
-      const double y1i = y1[(size_t)e * Np + i];
+      const double x = Xr[pe[j]];
+      const double q = Qr[j];
       double ym = q;
       if (i == j) ym += y1i;
-      ym = ym - y2[(size_t)e * Np + i] * (x * y2[(size_t)e * Np + j]);
+      ym = ym - y2i * (x * y2e[j]);
       v = ym + q;
       if (i == j) v += y1i;
     } else {
       v = (i == j) ? 1.0 : 0.0;
     }
-    A[e * n2 + idx] = v;
+    Ar[j] = v;
   }
 }
 
@@ -117,12 +125,12 @@ __global__ void k_form2(const double* __restrict__ X1, const int* __restrict__ p
 // of complex symmetric N x N matrices (N = half): Z -> (Z + Z^T)/2 is then
 // (Ye + P Ye^T P)/2 with P = diag(I, -I), i.e. the transposed element enters
 // with a minus sign when exactly one of i, j lies in the second half.
-__device__ __forceinline__ double yend_elem(const double* __restrict__ X2e, const int* __restrict__ pinve,
-                                            const double* __restrict__ We, const double* __restrict__ Wm, double h,
-                                            const double* __restrict__ y1e, const double* __restrict__ y2e, int N,
-                                            int Np, int i, int j) {
-  double t = We[(size_t)i * N + j];
-  if (i == j) t = t - Wm[(size_t)i * N + i];
+// (we = W_end[i][j], wm = W_mid[i][i], read by the caller)
+__device__ __forceinline__ double yend_elem(const double* __restrict__ X2e, const int* __restrict__ pinve, double we,
+                                            double wm, double h, const double* __restrict__ y1e,
+                                            const double* __restrict__ y2e, int Np, int i, int j) {
+  double t = we;
+  if (i == j) t = t - wm;
   double ye = h * t / 3.0;
   if (i == j) ye += y1e[i];
   const double x = X2e[(size_t)i * Np + pinve[j]];
@@ -140,34 +148,38 @@ struct NextSector {
   double* A;         // nullptr: write Y instead
 };
 
-__device__ __forceinline__ double next_a1(const NextSector& n, const double* __restrict__ Ws,
-                                          const double* __restrict__ Wm, const double* __restrict__ y1e, int N,
-                                          int i, int j, double v) {
+// (ws = the next sector's W_start[i][j], wm = its W_mid[i][i], read by the caller)
+__device__ __forceinline__ double next_a1(const NextSector& n, double ws, double wm, const double* __restrict__ y1e,
+                                          int N, int i, int j, double v) {
 #pragma clang fp contract(off)
   if (i >= N || j >= N) return (i == j) ? 1.0 : 0.0;
-  double t = Ws[(size_t)i * N + j];
-  if (i == j) t = t - Wm[(size_t)i * N + i];
+  double t = ws;
+  if (i == j) t = t - wm;
   const double qs = n.h * t / 3.0;
   double a = v + qs;
   if (i == j) a += y1e[i];
   return a;
 }
 
-// One workgroup per pair of mirror 32 x 32 tiles (ti <= tj): both tiles of Ye
-// are formed once into LDS and both tiles of Y written, so X2 is read once.
+// One workgroup per pair of mirror 32 x 32 tiles (ti <= tj) and energy: both tiles
+// of Ye are formed once into LDS and both tiles of Y written, so X2 is read once.
 // With nx.A set, the next sector's A1 is written instead of Y (k_form1 fused).
+// (Energy-major grids and groups of energies per workgroup sharing register-held
+// W tiles were measured and were not faster: 280 and 500 us against 269 us at
+// N = 800 x 64.)
 __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, const int* __restrict__ pinv,
-                                               const double* __restrict__ We, const double* __restrict__ Wm,
+                                               const double* __restrict__ We0, const double* __restrict__ Wm0,
                                                double h, const double* __restrict__ y1, const double* __restrict__ y2,
                                                int N, int Np, int half, double* __restrict__ Y, int item0, int per,
                                                long long gstride, NextSector nx) {
   __shared__ double E1[32][33];
   __shared__ double E2[32][33];
   const int e = blockIdx.y;
-  We += block_off(item0, e, per, gstride);
-  Wm += block_off(item0, e, per, gstride);
-  const double* nWs = nx.Ws + (nx.A ? block_off(item0, e, per, gstride) : 0);
-  const double* nWm = nx.Wm + (nx.A ? block_off(item0, e, per, gstride) : 0);
+  const size_t boff = block_off(item0, e, per, gstride);
+  const double* We = We0 + boff;
+  const double* Wm = Wm0 + boff;
+  const double* nWs = nx.Ws + (nx.A ? boff : 0);
+  const double* nWm = nx.Wm + (nx.A ? boff : 0);
   const double* ny1 = nx.y1 + (size_t)e * Np;
   double* Out = nx.A ? nx.A : Y;
   // pair index -> (ti, tj), 0 <= ti <= tj
@@ -186,10 +198,16 @@ __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, co
   const int ty = threadIdx.x >> 5;  // 0..7
   for (int r = ty; r < 32; r += 8) {
     const int i = ti * 32 + r, j = tj * 32 + tx;
-    E1[r][tx] = (i < N && j < N) ? yend_elem(X2e, pinve, We, Wm, h, y1e, y2e, N, Np, i, j) : 0.0;
+    E1[r][tx] = (i < N && j < N)
+                    ? yend_elem(X2e, pinve, We[(size_t)i * N + j], (i == j) ? Wm[(size_t)i * N + i] : 0.0, h, y1e,
+                                y2e, Np, i, j)
+                    : 0.0;
     if (mirror) {
       const int i2 = tj * 32 + r, j2 = ti * 32 + tx;
-      E2[r][tx] = (i2 < N && j2 < N) ? yend_elem(X2e, pinve, We, Wm, h, y1e, y2e, N, Np, i2, j2) : 0.0;
+      E2[r][tx] = (i2 < N && j2 < N)
+                      ? yend_elem(X2e, pinve, We[(size_t)i2 * N + j2], (i2 == j2) ? Wm[(size_t)i2 * N + i2] : 0.0, h,
+                                  y1e, y2e, Np, i2, j2)
+                      : 0.0;
     }
   }
   __syncthreads();
@@ -202,7 +220,10 @@ __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, co
       const bool flip = half > 0 && ((i < half) != (j < half));
       v = 0.5 * (flip ? a - t : a + t);
     }
-    Out[e * n2 + (size_t)i * Np + j] = nx.A ? next_a1(nx, nWs, nWm, ny1, N, i, j, v) : v;
+    Out[e * n2 + (size_t)i * Np + j] =
+        nx.A ? next_a1(nx, (i < N && j < N) ? nWs[(size_t)i * N + j] : 0.0,
+                       (i < N && i == j) ? nWm[(size_t)i * N + i] : 0.0, ny1, N, i, j, v)
+             : v;
     if (mirror) {
       const int i2 = tj * 32 + r, j2 = ti * 32 + tx;
       double v2 = 0.0;
@@ -211,7 +232,10 @@ __global__ __launch_bounds__(256) void k_form3(const double* __restrict__ X2, co
         const bool flip = half > 0 && ((i2 < half) != (j2 < half));
         v2 = 0.5 * (flip ? a - t : a + t);
       }
-      Out[e * n2 + (size_t)i2 * Np + j2] = nx.A ? next_a1(nx, nWs, nWm, ny1, N, i2, j2, v2) : v2;
+      Out[e * n2 + (size_t)i2 * Np + j2] =
+          nx.A ? next_a1(nx, (i2 < N && j2 < N) ? nWs[(size_t)i2 * N + j2] : 0.0,
+                         (i2 < N && i2 == j2) ? nWm[(size_t)i2 * N + i2] : 0.0, ny1, N, i2, j2, v2)
+               : v2;
     }
   }
 }
@@ -607,7 +631,7 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
         TICC_RETURN_IF(invert(a_in[g], a_alt[g], &r1, &pv1));
         // A2 goes where neither A1's inverse nor (mixed) A1 itself is needed again
         double* other = mixed ? a_alt[g] : ((r1 == A0) ? A1 : A0);
-        hipLaunchKernelGGL(k_form2, dim3(gelem, ne), dim3(256), 0, st, r1, pv1, Qbuf + q * n2, y1, y2, N, Np, other,
+        hipLaunchKernelGGL(k_form2, dim3((Np + 3) / 4, ne), dim3(256), 0, st, r1, pv1, Qbuf + q * n2, y1, y2, N, Np, other,
                            e0, per_block, qstride);
         TICC_CHECK_LAUNCH();
         double* r2 = nullptr;
