@@ -136,3 +136,19 @@ def test_propagate_buckets_validates_the_plan_before_device_work():
         propagate_buckets(hams, [0.1], (), None, (Bucket(32, (0, 1)),))
     with pytest.raises(ValueError, match="lanes"):
         propagate_buckets(hams, [0.1], (), None, (Bucket(64, (0, 1)),), lanes=3)
+
+
+def test_captured_buckets_validate_before_device_work():
+    from pyticc_amd import Propagation
+    from pyticc_amd.propagation.buckets import CapturedBuckets
+
+    hams = [_StubHam(20), _StubHam(40)]
+    plan = (Bucket(64, (0, 1)),)
+    with pytest.raises(NotImplementedError, match="FP64 inelastic"):
+        CapturedBuckets(hams, [0.1], ("s",), Propagation(device="gpu", precision="mixed"), plan)
+    with pytest.raises(NotImplementedError, match="FP64 inelastic"):
+        CapturedBuckets(hams, [0.1], ("s",), Propagation(device="gpu", mode="capture"), plan)
+    with pytest.raises(RuntimeError, match="no radial windows"):
+        CapturedBuckets(hams, [0.1], (), Propagation(device="gpu"), plan)
+    with pytest.raises(ValueError, match="exactly once"):
+        CapturedBuckets(hams, [0.1], ("s",), Propagation(device="gpu"), (Bucket(64, (1,)),))
