@@ -12,7 +12,10 @@ from pyticc_amd.scattering.hamiltonian import build_hamiltonian
 
 
 def oracle_problem(workload):
-    """Prepare the PES grid and the oracle Problem for a synthetic workload."""
+    """Prepare the PES grid and the oracle Problem for a synthetic workload.  The
+    Problem takes the product's setup arrays (channels, U, B_K rows, PES grid); for
+    the benchmark systems tests/test_oracle_setup.py shows them equal to the
+    oracle's own restatement (oracle/setup.py)."""
     grid = prepare_potential(workload.system, workload.boundaries, workload.half_steps, **workload.quadrature)
     ham = build_hamiltonian(workload.system, grid)
     return grid, ham, hamiltonian_problem(ham, grid)

@@ -116,3 +116,53 @@ def test_diatom_diatom_rows_give_bispherical_matrix_elements(terms, parity):
                          * _reduced_Y(jX, l1, jXp) * _reduced_Y(jY, l2, jYp))
     assert basis.n_channel > 20 and np.max(np.abs(ref)) > 0.05
     np.testing.assert_allclose(V, ref, rtol=0, atol=1e-13)
+
+
+# -- the oracle's own rows (oracle/setup.py) against the same closed forms ----------------
+
+from oracle import setup as osetup  # noqa: E402
+
+
+def _oracle_contracted_dd(parity, pes):
+    ch = osetup.channel_list(osetup.rigid_levels(10.0 * CM2AU, 2), osetup.rigid_levels(7.3 * CM2AU, 3), 3, parity)
+    ctX, wX = osetup.gl_rule(-1.0, 1.0, 10)
+    ctY, wY = osetup.gl_rule(-1.0, 1.0, 10)
+    phi, wphi = osetup.gl_rule(0.0, np.pi, 24)
+    rows = osetup.rows_diatom_diatom(ch, ctX, wX, ctY, wY, phi, wphi)
+    g = np.meshgrid([1.0], [1.0], np.arccos(ctX), np.arccos(ctY), phi, indexing="ij")
+    V = pes(np.array([5.0]), np.stack([x.reshape(-1) for x in g]))
+    blocks = [(idx, re, im) for _, (idx, re, im) in sorted(rows.items())]
+    return ch, ocon.contract(blocks, len(ch), 1.0 / np.pi, V)[0]
+
+
+@pytest.mark.parametrize("terms", [(1, 1, 2), (2, 1, 3), (2, 2, 4)])
+@pytest.mark.parametrize("parity", [1, -1])
+def test_oracle_diatom_diatom_rows_give_bispherical_matrix_elements(terms, parity):
+    l1, l2, lam = terms
+    ch, V = _oracle_contracted_dd(parity, _bispherical(l1, l2, lam))
+    ref = np.zeros_like(V)
+    for a, (jX, jY, j12, K, _) in enumerate(ch):
+        for b, (jXp, jYp, j12p, Kp, _) in enumerate(ch):
+            if K != Kp:
+                continue
+            ref[a, b] = ((-1) ** (j12 - K) * _w3j(j12, lam, j12p, -K, 0, K)
+                         * np.sqrt((2 * j12 + 1) * (2 * j12p + 1) * (2 * lam + 1))
+                         * _w9j(jX, jXp, l1, jY, jYp, l2, j12, j12p, lam)
+                         * _reduced_Y(jX, l1, jXp) * _reduced_Y(jY, l2, jYp))
+    assert np.max(np.abs(ref)) > 0.05
+    np.testing.assert_allclose(V, ref, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("lam", [1, 2, 5])
+def test_oracle_atom_diatom_rows_give_gaunt_matrix_elements(lam):
+    ch = osetup.channel_list([(0, 0.0)], osetup.rigid_levels(10.0 * CM2AU, 6), 4, 1)
+    ct, wt = osetup.gl_rule(-1.0, 1.0, 24)
+    rows = osetup.rows_atom_diatom(ch, ct, wt)
+    V = eval_legendre(lam, ct)[None, :]
+    V = ocon.contract([(idx, re, None) for _, (idx, re) in sorted(rows.items())], len(ch), 1.0, V)[0]
+    ref = np.zeros_like(V)
+    for a, (_, j, _, K, _) in enumerate(ch):
+        for b, (_, jp, _, Kp, _) in enumerate(ch):
+            if K == Kp:
+                ref[a, b] = (-1) ** K * np.sqrt((2 * j + 1) * (2 * jp + 1)) * _w3j(j, lam, jp, 0, 0, 0) * _w3j(j, lam, jp, -K, 0, K)
+    np.testing.assert_allclose(V, ref, rtol=0, atol=1e-13)
