@@ -25,7 +25,9 @@ parity: the timed run's S at the energies of the committed N = 800 fixture
 (tests/golden/smat_rotor_rotor_N800.npz, written by the oracle) against it.
 cpu_baseline: the NumPy oracle (a restatement of the reference algorithm) on
 the same system, all 64 energies over the first sectors, about 30 s of work in
-three runs, on this host's cores (CPU model and BLAS library stated).
+three runs, on this host's cores (CPU model and BLAS library stated); its
+`config1` record runs the N = 66 atom + rotor system in full on both, with the
+S matrices compared.
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu-baseline]
        [--max-sectors M]   (first M sectors only: a short run for counter profiles;
@@ -118,6 +120,46 @@ def cpu_baseline(workload, grid, min_seconds: float = 30.0, max_sectors: int = 6
         f"BLAS threads={threads}; CPU {_cpu_model()}; BLAS {blas}",
         "cpu_model": _cpu_model(),
         "blas": blas,
+    }
+
+
+def config1_baseline() -> dict:
+    """The atom + rotor system (N = 66, J = 10, p = +1, 20 energies, 1350 sectors) in
+    full: one GPU solve (after one untimed) and one NumPy oracle solve on this host,
+    with the S matrices compared."""
+    import threadpoolctl
+    import torch
+
+    import pyticc_amd as ticc
+    from oracle import pipeline as opipe
+    from pyticc_amd import synthetic
+    from tests.helpers import oracle_problem
+
+    w = synthetic.atom_rotor()
+    grid, _, prob = oracle_problem(w)
+    prop = ticc.Propagation(device="gpu")
+    ticc.solve(w.system, w.energies, grid, prop)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    res = ticc.solve(w.system, w.energies, grid, prop)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t
+    t = time.perf_counter()
+    _, _, _, S_ref = opipe.solve(prob, w.energies)
+    cpu_s = time.perf_counter() - t
+    steps = w.energies.size * len(grid.sectors)
+    threads = max((d.get("num_threads", 1) for d in threadpoolctl.threadpool_info()), default=1)
+    return {
+        "workload": f"atom_rotor exact CC in full: N={w.system.basis.n_channel}, J=10, p=+1, {w.energies.size} energies, "
+        f"{len(grid.sectors)} sectors 3-30 bohr (contraction + sweep + matching)",
+        "gpu_value": steps / gpu_s,
+        "cpu_value": steps / cpu_s,
+        "unit": "energy.R-steps/s",
+        "gpu_seconds": gpu_s,
+        "cpu_seconds": cpu_s,
+        "cpu_threads": int(threads),
+        "kind": "port",
+        "max_abs_dS_gpu_vs_oracle": float(max(np.max(np.abs(a - b)) for a, b in zip(res.Smat, S_ref) if b.size)),
     }
 
 
@@ -244,6 +286,12 @@ def main() -> int:
         t = time.perf_counter()
         cpu = cpu_baseline(w, grid)
         _log(f"cpu baseline {time.perf_counter() - t:.1f} s: {cpu['value']:.2f} {cpu['unit']}")
+        if args.max_sectors == 0:
+            t = time.perf_counter()
+            cpu["config1"] = config1_baseline()
+            _log(f"config 1 (N = 66 in full) {time.perf_counter() - t:.1f} s: GPU {cpu['config1']['gpu_value']:.0f}, "
+                 f"CPU {cpu['config1']['cpu_value']:.0f} {cpu['unit']}, "
+                 f"max|dS| {cpu['config1']['max_abs_dS_gpu_vs_oracle']:.1e}")
 
     peak = FP64_MFMA_PEAK_TFLOPS if args.precision == "fp64" else FP32_MFMA_PEAK_TFLOPS
     line = {
