@@ -10,10 +10,13 @@ LDMD sweep of all 64 energies, and the fused BF->SF + K->S matching.
 value = energies x sectors / seconds per step (energy.R-steps/s).
 roofline: the sweep's dominant kernel, k_gj_fused -- the rank-64 Gauss-Jordan
 update of a pair of block steps (FP64 MFMA) with the next pair's panel chain in
-its first workgroups -- timed live with HIP events on the stream it runs on,
+its first workgroups -- timed live with HIP events on the stream it runs on
+(every launch of the first sectors of the timed region, both energy groups),
 priced against the FP64 matrix peak as the LDMD sweep's accounting does (4 N^3
 FP64 flop per energy.R-step, two N x N inversions): per launch on m matrices
-the update does 2 Np^2 64 m FP64 flop.  Its algorithmic bytes, m (2 Np^2 + 64 Np
+the update does 2 Np^2 64 m FP64 flop; the rate is the timed launches' flop over
+the union of their intervals (the two energy groups' launches run side by side,
+and the time they share is counted once).  Its algorithmic bytes, m (2 Np^2 + 64 Np
 + 128 Np) 8 (the update reads X, writes the partner buffer and reads the
 coefficient columns; the panel chain reads its look-ahead columns and writes
 its coefficients), and the achieved GB/s are reported beside it.  traffic: HBM
@@ -182,7 +185,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-every", type=int, default=8, help="time every k-th batched inversion's fused launches")
+    ap.add_argument("--profile-every", type=int, default=1, help="time every k-th batched inversion's fused launches")
+    ap.add_argument("--profile-launches", type=int, default=8192,
+                    help="time the fused launches of the timed region up to this count (both energy groups' "
+                    "launches of the same sectors, so that their overlap is measured)")
     ap.add_argument("--max-sectors", type=int, default=0, help="propagate only the first M sectors (profiling runs)")
     ap.add_argument("--workload", choices=("rotor_rotor", "batching"), default="rotor_rotor")
     ap.add_argument("--blocks", type=int, default=256, help="batching workload: number of blocks")
@@ -242,7 +248,7 @@ def main() -> int:
         _log(f"warmup {i}: {time.perf_counter() - t:.2f} s")
 
     lib = _backend.load()
-    _backend.call("ticc_profile_update_begin", args.profile_every, 1 << 18)
+    _backend.call("ticc_profile_update_begin", args.profile_every, args.profile_launches)
     torch.cuda.synchronize()
     t = time.perf_counter()
     for i in range(args.steps):
@@ -251,10 +257,12 @@ def main() -> int:
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t
     tot_ms = np.zeros(1)
+    uni_ms = np.zeros(1)
     cnt = np.zeros(1, dtype=np.int32)
     tot_bytes = np.zeros(1)
     tot_flop = np.zeros(1)
-    _backend.check(lib.ticc_profile_update_end(tot_ms.ctypes.data_as(_backend._dp), cnt.ctypes.data_as(_backend._ip),
+    _backend.check(lib.ticc_profile_update_end(tot_ms.ctypes.data_as(_backend._dp), uni_ms.ctypes.data_as(_backend._dp),
+                                               cnt.ctypes.data_as(_backend._ip),
                                                tot_bytes.ctypes.data_as(_backend._dp),
                                                tot_flop.ctypes.data_as(_backend._dp)))
     ms_per_step = 1e3 * elapsed / args.steps
@@ -264,8 +272,13 @@ def main() -> int:
     avg_ms = float(tot_ms[0]) / max(1, n_timed)
     bytes_per_launch = float(tot_bytes[0]) / max(1, n_timed)
     flop_per_launch = float(tot_flop[0]) / max(1, n_timed)
-    achieved_gbs = float(tot_bytes[0]) / (float(tot_ms[0]) * 1e-3) / 1e9 if n_timed else None
-    achieved_tf = float(tot_flop[0]) / (float(tot_ms[0]) * 1e-3) / 1e12 if n_timed else None
+    # The kernel's rate: the timed launches' work over the time in which at
+    # least one of them ran (the union of their intervals).  With one energy
+    # group that is the summed launch time; with two, each group's launches run
+    # beside the other group's, and the union counts the shared time once.
+    busy_ms = float(uni_ms[0])
+    achieved_gbs = float(tot_bytes[0]) / (busy_ms * 1e-3) / 1e9 if n_timed else None
+    achieved_tf = float(tot_flop[0]) / (busy_ms * 1e-3) / 1e12 if n_timed else None
     S = unpack_smat(S_re, S_im, n_open)
     defect = max(float(np.max(np.abs(s.conj().T @ s - np.eye(s.shape[0])))) for s in S if s.size)
     _log(f"fused kernel: {n_timed} timed launches, avg {avg_ms * 1e3:.1f} us; unitarity defect {defect:.2e}")
@@ -330,6 +343,9 @@ def main() -> int:
             "flop_per_launch": flop_per_launch,
             "avg_launch_us": avg_ms * 1e3,
             "timed_launches": n_timed,
+            "busy_ms": busy_ms,
+            "summed_launch_ms": float(tot_ms[0]),
+            "mean_concurrent_launches": float(tot_ms[0]) / busy_ms if busy_ms > 0 else None,
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "hbm_gbs": achieved_gbs,
             "hbm_frac": None if achieved_gbs is None else achieved_gbs / HBM_PEAK_GBS,

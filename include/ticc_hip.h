@@ -387,11 +387,15 @@ int ticc_set_sweep_walk(int fused_target, int update_target);
  * rank-64 update beside the next pair's panel chain).  Between begin and end,
  * every sample_every-th batched inversion has each of its fused launches
  * bracketed by an event pair on the stream it runs on (at most max_samples
- * pairs).  end synchronises the device and returns the summed kernel time, the
- * number of timed launches and their summed algorithmic bytes and FP64 flop.
+ * pairs).  end synchronises the device and returns the summed kernel time,
+ * the length of the union of the timed launches' intervals on the device
+ * clock (the time in which at least one of them ran: the sum when they never
+ * overlap, less when launches of energy groups on separate streams run side by
+ * side), the number of timed launches and their summed algorithmic bytes and
+ * FP64 flop.
  */
 int ticc_profile_update_begin(int sample_every, int max_samples);
-int ticc_profile_update_end(double* total_ms, int* count, double* total_bytes, double* total_flop);
+int ticc_profile_update_end(double* total_ms, double* union_ms, int* count, double* total_bytes, double* total_flop);
 
 #ifdef __cplusplus
 }

@@ -70,7 +70,7 @@ _SIGNATURES = {
     "ticc_invert_batched_noalloc": ([_c_int, _c_int, _vp, _vp], _c_int),
     "ticc_set_sweep_walk": ([_c_int, _c_int], _c_int),
     "ticc_profile_update_begin": ([_c_int, _c_int], _c_int),
-    "ticc_profile_update_end": ([_dp, _ip, _dp, _dp], _c_int),
+    "ticc_profile_update_end": ([_dp, _dp, _ip, _dp, _dp], _c_int),
     "ticc_dgemm_batched": ([_c_int, _c_int, _c_int, _c_d, _vp, _c_ll, _vp, _c_ll, _c_d, _vp, _c_ll, _c_int, _vp], _c_int),
     "ticc_mlp_gemm": ([_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "ticc_mlp_features": ([_c_int, _c_int, _vp, _c_d, _vp, _vp, _vp], _c_int),

@@ -39,7 +39,7 @@ int get_smat(int nE, int N, const double* Y, const double* B, double Rmatch, con
 int profile_begin(int every, int max_pairs);
 int mixed_stats(long long* inversions, long long* fallbacks, int reset);
 int gj_set_walk(int fused_target, int update_target);
-int profile_end(double* total_ms, int* count, double* total_bytes, double* total_flop);
+int profile_end(double* total_ms, double* union_ms, int* count, double* total_bytes, double* total_flop);
 int bessel_eval(int n, const double* ell, const double* x, double* out, const double* nu, double* kout,
                 hipStream_t stream);
 }  // namespace ticc
@@ -486,9 +486,10 @@ int ticc_profile_update_begin(int sample_every, int max_samples) {
   return ticc::profile_begin(sample_every, max_samples);
 }
 
-int ticc_profile_update_end(double* total_ms, int* count, double* total_bytes, double* total_flop) {
-  if (!total_ms || !count || !total_bytes || !total_flop) return fail(TICC_ERR_VALUE, "profile: null pointer argument");
-  return ticc::profile_end(total_ms, count, total_bytes, total_flop);
+int ticc_profile_update_end(double* total_ms, double* union_ms, int* count, double* total_bytes, double* total_flop) {
+  if (!total_ms || !union_ms || !count || !total_bytes || !total_flop)
+    return fail(TICC_ERR_VALUE, "profile: null pointer argument");
+  return ticc::profile_end(total_ms, union_ms, count, total_bytes, total_flop);
 }
 
 }  // extern "C"

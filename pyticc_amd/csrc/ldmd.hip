@@ -409,13 +409,15 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   // batch item b belongs to block b / per; each block's Q_mid chunk holds qc sectors
   const int per_block = nE / n_block;
   const int qc = n_block == 1 ? QCHUNK : (QCHUNK / n_block > 0 ? QCHUNK / n_block : 1);
-  // Energy groups may run on their own streams (TICC_SWEEP_GROUPS): with two,
-  // one group's serial pieces (its first panel chain, look-ahead updates, form
-  // kernels) run beside the other group's fused launches, 3-5% faster at
-  // N = 800 and 400 with 64 energies (three or four groups are slower).  One
-  // group is the default, so that each fused launch runs alone and its timing
-  // measures the kernel itself.
-  int G = 1;
+  // Energy groups run on their own streams: with two, one group's serial pieces
+  // (its first panel chain, look-ahead updates, form kernels) run beside the
+  // other group's fused launches, 3-5% faster at N = 800 and 400 with 64
+  // energies, results bit for bit those of one group (three groups are 35%
+  // slower at N = 800).  Two groups are the default of the eager FP64
+  // inelastic sweep of one block with at least 16 energies and Np >= 256;
+  // TICC_SWEEP_GROUPS overrides it.
+  const bool mixed = refine >= 0;
+  int G = (!mixed && !as && half == 0 && n_block == 1 && nE >= 16 && Np >= 256) ? 2 : 1;
   if (const char* env = getenv("TICC_SWEEP_GROUPS")) {  // tuning knob: 1..MAX_GROUPS
     const int v = atoi(env);
     if (v >= 1 && v <= MAX_GROUPS) G = v < nE ? v : nE;
@@ -440,7 +442,6 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   // residual I - A X on the FP64 MFMA from the FP64 A, the correction X (I - A X)
   // on the FP32 MFMA; everything else (forms, reference values, Q_mid, Y) stays
   // FP64.  refine < 0: the FP64 sweep.
-  const bool mixed = refine >= 0;
   struct Scratch {
     size_t A0, A1, P, src, piv, perm, pinv, y1, y2;
     size_t F0, F1, X, T32, X32[2], ipinv, XB, rho, mask;  // mixed sweep only
