@@ -40,6 +40,26 @@ def contract(blocks, n_channel: int, norm: float, V: np.ndarray) -> np.ndarray:
     return out
 
 
+def contract_split(blocks, n_channel: int, norm: float, V: np.ndarray) -> np.ndarray:
+    """contract() with each angular sum taken in two halves (the first half of the
+    quadrature points, then the rest) and the partial matrices added: the same
+    arithmetic in another FP64 summation order, as the device's tiled contraction
+    sums in its own order (used to bound what an end-to-end comparison resolves)."""
+    V = np.atleast_2d(np.asarray(V, dtype=np.float64))
+    out = np.zeros((V.shape[0], n_channel, n_channel))
+    for idx, Br, Bi in blocks:
+        idx = np.asarray(idx)
+        h = Br.shape[1] // 2
+        for r in range(V.shape[0]):
+            v = V[r]
+            X = (Br[:, :h] * v[None, :h]) @ Br[:, :h].T + (Br[:, h:] * v[None, h:]) @ Br[:, h:].T
+            if Bi is not None:
+                X += (Bi[:, :h] * v[None, :h]) @ Bi[:, :h].T + (Bi[:, h:] * v[None, h:]) @ Bi[:, h:].T
+            X *= norm
+            out[r][np.ix_(idx, idx)] = 0.5 * (X + X.T)
+    return out
+
+
 def w_base(R: np.ndarray, mu: float, E_int: np.ndarray, U: np.ndarray, Vmat: np.ndarray) -> np.ndarray:
     """W_base(R) = U/R^2 + 2 mu V(R) + 2 mu diag(E_int)  (runner.py:265-271)."""
     R = np.asarray(R, dtype=np.float64)

@@ -112,6 +112,75 @@ def sector_step_inverse(Y, h, ref, Qs, Qm, Qe):
     return 0.5 * (Yend + Yend.T)
 
 
+def gauss_jordan_inverse(A, nb: int = 32):
+    """Inverse by blocked Gauss-Jordan elimination with partial pivoting: columns
+    in blocks of nb; in each block the pivot of every column is the largest |a|
+    among the rows at positions >= the column (the first on ties, as LAPACK's
+    idamax), found by in-place Gauss-Jordan on the block's column panel, and the
+    block's transformation is then applied to the other columns at once
+    (pivot rows B^-1 R, other rows A_i - A_i[panel] B^-1 R).  Row interchanges
+    make the result the inverse of the row-permuted matrix; its columns are
+    gathered back at the end.  Another FP64 ordering of the inversions in
+    logd.py:168-194 (the device inverts this way, with its own blocking)."""
+    X = np.array(A, dtype=np.float64, copy=True)
+    n = X.shape[0]
+    perm = np.arange(n)
+    for k0 in range(0, n, nb):
+        c = slice(k0, min(n, k0 + nb))
+        w = c.stop - k0
+        P = X[:, c].copy()
+        rows = np.arange(n)  # panel row order: position -> row of X
+        for j in range(w):
+            k = k0 + j
+            p = k + int(np.argmax(np.abs(P[k:, j])))
+            if p != k:
+                P[[k, p]] = P[[p, k]]
+                rows[[k, p]] = rows[[p, k]]
+            piv = P[k, j]
+            P[k] /= piv
+            P[k, j] = 1.0 / piv
+            f = P[:, j].copy()
+            f[k] = 0.0
+            P -= np.outer(f, P[k])
+            P[:, j] = -f / piv
+            P[k, j] = 1.0 / piv
+        moved = np.nonzero(rows != np.arange(n))[0]
+        X[moved] = X[rows[moved]]
+        perm = perm[rows]
+        R = X[k0:c.stop].copy()
+        # (the products also cover the panel columns, which P then overwrites)
+        X[:k0] += P[:k0] @ R
+        X[c.stop:] += P[c.stop:] @ R
+        X[k0:c.stop] = P[k0:c.stop] @ R
+        X[:, c] = P
+    out = np.empty_like(X)
+    out[:, perm] = X
+    return out
+
+
+def sector_step_gj(Y, h, ref, Qs, Qm, Qe):
+    """The sector with both inverses formed by blocked Gauss-Jordan
+    (gauss_jordan_inverse) and then scaled: a third FP64 ordering of
+    logd.py:168-194, of the same algorithm class as the device's, whose distance
+    from the solve-based step bounds what an end-to-end comparison can resolve."""
+    y1, y2 = reference_values(h, ref)
+    n = Y.shape[-1]
+    idx = np.arange(n)
+    A1 = Y + Qs
+    A1[idx, idx] += y1
+    X1 = gauss_jordan_inverse(A1) * y2[None, :]
+    Ymid = Qm.copy()
+    Ymid[idx, idx] += y1
+    Ymid -= y2[:, None] * X1
+    A2 = Ymid + Qm
+    A2[idx, idx] += y1
+    X2 = gauss_jordan_inverse(A2) * y2[None, :]
+    Yend = Qe.copy()
+    Yend[idx, idx] += y1
+    Yend -= y2[:, None] * X2
+    return 0.5 * (Yend + Yend.T)
+
+
 def propagate(Y0, energies, mu, half_steps, Wb_start, Wb_mid, Wb_end):
     """Energy batch x sector scan (logd.py:288-330, 340-398).
 

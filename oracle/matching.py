@@ -77,30 +77,38 @@ def reference_functions(energy, Rmatch, mu, E_int, L):
     return op, cl, J, N, Jp, Np, G
 
 
-def open_reaction_matrix(Y, op, cl, J, N, Jp, Np, G):
+def _solve(A, B, inverse):
+    """A^-1 B by a linear solve, or (inverse=True) through the explicit inverse --
+    two FP64 orderings of the same step."""
+    return np.linalg.inv(A) @ B if inverse else np.linalg.solve(A, B)
+
+
+def open_reaction_matrix(Y, op, cl, J, N, Jp, Np, G, inverse: bool = False):
     """Y_eff = Y_oo - Y_oc (Y_cc - G)^-1 Y_co; K = -(Y_eff N - N')^-1 (Y_eff J - J') (smatrix.py:86-132)."""
     Yo = Y[np.ix_(op, op)]
     if cl.size:
         Yoc = Y[np.ix_(op, cl)]
         Yco = Y[np.ix_(cl, op)]
         Ycc = Y[np.ix_(cl, cl)] - np.diag(G)
-        Yo = Yo - Yoc @ np.linalg.solve(Ycc, Yco)
+        Yo = Yo - Yoc @ _solve(Ycc, Yco, inverse)
     left = Yo * N[None, :] - np.diag(Np)
     right = Yo * J[None, :] - np.diag(Jp)
-    return np.asarray(-np.linalg.solve(left, right), dtype=np.complex128)
+    return np.asarray(-_solve(left, right, inverse), dtype=np.complex128)
 
 
-def get_smat(Y, Rmatch, energies, mu, E_int, L):
-    """Ragged tuple of open-open S matrices (smatrix.py:139-223)."""
+def get_smat(Y, Rmatch, energies, mu, E_int, L, inverse: bool = False):
+    """Ragged tuple of open-open S matrices (smatrix.py:139-223).  inverse=True
+    forms every solve through an explicit inverse: another FP64 ordering, whose
+    distance from the solve-based S is the matching step's own noise floor."""
     out = []
     for e, energy in enumerate(np.asarray(energies, dtype=np.float64)):
         op, cl, J, N, Jp, Np, G = reference_functions(float(energy), Rmatch, mu, E_int, L)
         if op.size == 0:
             out.append(np.empty((0, 0), dtype=np.complex128))
             continue
-        K = open_reaction_matrix(Y[e], op, cl, J, N, Jp, Np, G)
+        K = open_reaction_matrix(Y[e], op, cl, J, N, Jp, Np, G, inverse)
         eye = np.eye(op.size, dtype=np.complex128)
-        out.append(np.linalg.solve(eye + 1.0j * K, eye - 1.0j * K))
+        out.append(_solve(eye + 1.0j * K, eye - 1.0j * K, inverse))
     return tuple(out)
 
 

@@ -43,31 +43,32 @@ class Problem:
         return self.U.shape[0]
 
 
-def w_points(p: Problem, rows: np.ndarray) -> np.ndarray:
-    """W_base at the given indices of the ordered radial points."""
+def w_points(p: Problem, rows: np.ndarray, contract=contraction.contract) -> np.ndarray:
+    """W_base at the given indices of the ordered radial points (``contract``: the
+    contraction's summation order, contraction.contract by default)."""
     R = grid.radial_points(p.sectors)[rows]
-    Vm = contraction.contract(p.blocks, p.n_channel, p.norm, p.V[rows])
+    Vm = contract(p.blocks, p.n_channel, p.norm, p.V[rows])
     return contraction.w_base(R, p.mu, p.E_int, p.U, Vm)
 
 
 def propagate(p: Problem, energies, sectors_limit: int | None = None, mode: str = "inelastic",
-              steps=(ldmd.sector_step,)) -> list:
+              steps=(ldmd.sector_step,), contract=contraction.contract) -> list:
     """Y_BF after all (or the first ``sectors_limit``) sectors, shape (n_E, N, N).
 
     ``steps``: the sector-step orderings to run side by side on the same W points
     (default: the solve-based step alone, returned as an array); with several,
-    a list of one Y per ordering."""
+    a list of one Y per ordering.  ``contract``: the contraction's summation order."""
     E = np.asarray(energies, dtype=np.float64)
     n_s = len(p.sectors) if sectors_limit is None else min(sectors_limit, len(p.sectors))
     hs = grid.half_steps_of(p.sectors)
-    W_prev = w_points(p, np.array([0]))[0]
+    W_prev = w_points(p, np.array([0]), contract)[0]
     eye = np.eye(p.n_channel)
     W0 = W_prev[None] - 2.0 * p.mu * E[:, None, None] * eye
     Y0 = ldmd.initialize_inelastic(W0) if mode == "inelastic" else ldmd.initialize_capture(W0)
     Ys = [Y0.copy() for _ in steps]
     shift = 2.0 * p.mu * E
     for s in range(n_s):
-        Wm, We = w_points(p, np.array([2 * s + 1, 2 * s + 2]))
+        Wm, We = w_points(p, np.array([2 * s + 1, 2 * s + 2]), contract)
         ref, Qs, Qm, Qe = ldmd.correction_matrices(hs[s], W_prev, Wm, We)
         for Y, step in zip(Ys, steps):
             for e in range(E.size):
@@ -76,11 +77,12 @@ def propagate(p: Problem, energies, sectors_limit: int | None = None, mode: str 
     return Ys[0] if len(steps) == 1 else Ys
 
 
-def match(p: Problem, Y, energies):
-    """(B, L, S tuple) of propagated BF log derivatives (finalize.py:32-98)."""
+def match(p: Problem, Y, energies, inverse: bool = False):
+    """(B, L, S tuple) of propagated BF log derivatives (finalize.py:32-98);
+    inverse=True: the matching's solves through explicit inverses."""
     B, L = matching.bmat_bf_to_sf(p.U, p.ladder, p.j_couple, p.Jtot)
     Ysf = matching.transform_bf_to_sf(Y, B)
-    return B, L, matching.get_smat(Ysf, p.sectors[-1][1], energies, p.mu, p.E_int, L)
+    return B, L, matching.get_smat(Ysf, p.sectors[-1][1], energies, p.mu, p.E_int, L, inverse)
 
 
 def solve(p: Problem, energies, mode: str = "inelastic"):

@@ -116,6 +116,33 @@ __device__ __forceinline__ void wave_argmax_dpp(double& v, int& p) {
   p = rp[0];
 }
 
+// One step of the panel's cross-wave winner reduction: take the record of the
+// lane CTRL's row rotation brings when it is better (larger key, smaller
+// position on ties).
+template <int CTRL>
+__device__ __forceinline__ void panel_winner_step(unsigned long long& k, int& p, int& w) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(k & 0xffffffffull), CTRL, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(k >> 32), CTRL, 0xF, 0xF, false);
+  const int op = __builtin_amdgcn_update_dpp(0, p, CTRL, 0xF, 0xF, false);
+  const int ow = __builtin_amdgcn_update_dpp(0, w, CTRL, 0xF, 0xF, false);
+  const unsigned long long ok = ((unsigned long long)hi << 32) | lo;
+  const bool tb = (ok > k) | ((ok == k) & (op < p));
+  k = tb ? ok : k;
+  p = tb ? op : p;
+  w = tb ? ow : w;
+}
+
+template <typename T>
+__device__ __forceinline__ T readlane_t(T v, int lane);
+template <>
+__device__ __forceinline__ double readlane_t<double>(double v, int lane) {
+  return readlane_f64(v, lane);
+}
+template <>
+__device__ __forceinline__ float readlane_t<float>(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 // Panel: one workgroup per matrix; thread t owns rows t + THREADS q
 // (q < RPT, Np <= THREADS RPT) with their NB panel values in registers and
 // its rows' current positions.  Per column: each wave reduces its best
@@ -181,6 +208,7 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     // This thread's candidate: largest |a| among its rows at positions >= c,
     // first position on ties (NaN ranks as +inf so a column always has a pivot
     // and the NaN propagates to the result).
+    // (bitwise & and |: no short-circuit branches around the comparisons)
     double cv = -1.0;
     int cp = 0x7fffffff;
     int cq = -1;
@@ -188,11 +216,22 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     for (int q = 0; q < RPT; ++q) {
       double a = fabs((double)v[q][u]);
       a = (a != a) ? __builtin_inf() : a;
-      const bool take = (pos[q] >= c) && better(a, pos[q], cv, cp);
+      const bool take = (pos[q] >= c) & ((a > cv) | ((a == cv) & (pos[q] < cp)));
       cv = take ? a : cv;
       cp = take ? pos[q] : cp;
       cq = take ? q : cq;
     }
+    // The candidate's signed value, its value in the next column and the
+    // reciprocal of the value, formed by every lane before the wave's winner is
+    // known, so that the division overlaps the reductions (only the winning
+    // lane's are published).
+    T pv = v[0][u], pnx = v[0][u + 1];
+#pragma unroll
+    for (int q = 1; q < RPT; ++q) {
+      pv = (cq == q) ? v[q][u] : pv;
+      pnx = (cq == q) ? v[q][u + 1] : pnx;
+    }
+    const T rinv = T(1) / pv;
     // Exact ordering key of the candidate: the bits of |a| plus one (monotone in
     // |a|), 0 for no candidate.  The wave maximum of its top 32 bits takes one
     // DPP max per step; only when several lanes share that maximum is the exact
@@ -228,50 +267,44 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     // own LDS slot, so the write needs no register select), its key, position,
     // row, the reciprocal of its pivot value and its value in the next column.
     if (lane == wl) {
-      T pv = 0, pn = 0;
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         if (cq == q) {
 #pragma unroll
           for (int l = 0; l < NB; ++l) s_vals[buf][wave][q][l] = v[q][l];
-          pv = v[q][u];
-          pn = v[q][u + 1];
         }
       }
       s_key[buf][wave] = key;
       s_pos[buf][wave] = cp;
       s_row[buf][wave] = t + THREADS * cq;
-      s_pinv[buf][wave] = T(1) / pv;
-      s_pnext[buf][wave] = pn;
+      s_pinv[buf][wave] = rinv;
+      s_pnext[buf][wave] = pnx;
     } else if (wl < 0 && lane == 0) {
       s_key[buf][wave] = 0ull;
       s_pos[buf][wave] = 0x7fffffff;
     }
     __syncthreads();
-    // Global winner: larger key, smaller position on ties (a pairwise tree).
-    unsigned long long kw[NW];
-    int pw[NW], ww[NW];
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      kw[w] = s_key[buf][w];
-      pw[w] = s_pos[buf][w];
-      ww[w] = w;
-    }
-#pragma unroll
-    for (int h = 1; h < NW; h <<= 1) {
-#pragma unroll
-      for (int w = 0; w + h < NW; w += 2 * h) {
-        const bool tb = (kw[w + h] > kw[w]) || (kw[w + h] == kw[w] && pw[w + h] < pw[w]);
-        kw[w] = tb ? kw[w + h] : kw[w];
-        pw[w] = tb ? pw[w + h] : pw[w];
-        ww[w] = tb ? ww[w + h] : ww[w];
-      }
-    }
-    const int bw = ww[0];
-    const int bp = pw[0];
-    const int pr = s_row[buf][bw];
-    const T inv = s_pinv[buf][bw];
-    const T pn = s_pnext[buf][bw];
+    // Global winner: larger key, smaller position on ties.  Lane l reads wave
+    // (l mod NW)'s record, so every NW consecutive lanes of a 16-lane row hold
+    // all records, and log2(NW) DPP row rotations leave the winner in every
+    // lane; its row, reciprocal and next-column value are then read from the
+    // lane that loaded its record.  (A wave without a candidate has key 0 and
+    // stale other fields, and never wins: some wave always has a candidate.)
+    const int wr = lane & (NW - 1);
+    unsigned long long kb = s_key[buf][wr];
+    int pb = s_pos[buf][wr];
+    int wb = wr;
+    const int rrow = s_row[buf][wr];
+    const T rpinv = s_pinv[buf][wr];
+    const T rpn = s_pnext[buf][wr];
+    panel_winner_step<0x121>(kb, pb, wb);  // row_ror:1
+    if constexpr (NW > 2) panel_winner_step<0x122>(kb, pb, wb);  // row_ror:2
+    if constexpr (NW > 4) panel_winner_step<0x124>(kb, pb, wb);  // row_ror:4
+    const int bw = __builtin_amdgcn_readfirstlane(wb);
+    const int bp = __builtin_amdgcn_readfirstlane(pb);
+    const int pr = __builtin_amdgcn_readlane(rrow, bw);
+    const T inv = readlane_t<T>(rpinv, bw);
+    const T pn = readlane_t<T>(rpn, bw);
     const T* prow = s_vals[buf][bw][pr / THREADS];
     // Gauss-Jordan step with the pivot row's scaling deferred: every other row
     // r takes a_rl -= (a_rj / p) p_l, a_rj = -a_rj / p; the pivot row keeps its

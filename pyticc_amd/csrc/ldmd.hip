@@ -412,12 +412,12 @@ int ldmd_propagate(int nE, int N, const double* E_host, double mu, int nS, const
   // Energy groups run on their own streams: with two, one group's serial pieces
   // (its first panel chain, look-ahead updates, form kernels) run beside the
   // other group's fused launches, 3-5% faster at N = 800 and 400 with 64
-  // energies, results bit for bit those of one group (three groups are 35%
-  // slower at N = 800).  Two groups are the default of the eager FP64
-  // inelastic sweep of one block with at least 16 energies and Np >= 256;
-  // TICC_SWEEP_GROUPS overrides it.
+  // energies (4.5% for the mixed-precision sweep), results bit for bit those
+  // of one group (three groups are 35% slower at N = 800).  Two groups are the
+  // default of the eager inelastic sweep of one block with at least 16
+  // energies and Np >= 256; TICC_SWEEP_GROUPS overrides it.
   const bool mixed = refine >= 0;
-  int G = (!mixed && !as && half == 0 && n_block == 1 && nE >= 16 && Np >= 256) ? 2 : 1;
+  int G = (!as && half == 0 && n_block == 1 && nE >= 16 && Np >= 256) ? 2 : 1;
   if (const char* env = getenv("TICC_SWEEP_GROUPS")) {  // tuning knob: 1..MAX_GROUPS
     const int v = atoi(env);
     if (v >= 1 && v <= MAX_GROUPS) G = v < nE ? v : nE;

@@ -22,6 +22,21 @@ def rotor_rotor_N800_reference(d, i):
 
 
 def n800_tolerance(d, i, bar=1.0e-10):
-    """Per-energy |dS| tolerance at full size: the 1e-10 bar, or twice the FP64 noise
-    floor recorded with the fixture where that floor exceeds half of the bar."""
+    """Per-energy |dS| tolerance at full size for S computed from a propagated Y
+    by the oracle's matching: the 1e-10 bar, or twice the FP64 noise floor of the
+    contraction and sweep recorded with the fixture (the largest distance between
+    the oracle and its other orderings) where that floor exceeds half of the bar."""
     return max(bar, 2.0 * float(d["floor"][i]))
+
+
+def n800_match_tolerance(d, i, bar=1.0e-10):
+    """Per-energy tolerance for two matchings of the same Y: the 1e-10 bar, or
+    twice the matching step's own floor recorded with the fixture."""
+    fm = float(d["floor_match"][i]) if "floor_match" in d.files else 0.0
+    return max(bar, 2.0 * fm)
+
+
+def n800_total_tolerance(d, i, bar=1.0e-10):
+    """Bound on the distance of a device S (its own sweep and matching) from the
+    fixture: the two parts' tolerances added."""
+    return n800_tolerance(d, i, bar) + n800_match_tolerance(d, i, bar)

@@ -15,13 +15,20 @@ Fixtures (NumPy .npz, no pickled objects):
                       p = +1, 1350 sectors 3-30 bohr) at 5 of its 20 energies.
   smat_rotor_rotor_N800.npz    oracle S-matrices of the two-rotor system (N = 800, J = 4, p = -1,
                       2000 sectors 4-40 bohr, 15 x 15 x 12 grid) at 6 of its 64 energies:
-                      every element of S at energies 4, 15 and 31 (17, 190 and 473 open
+                      every element of S at energies 4, 15 and 31 (17, 202 and 473 open
                       channels; the upper triangle, S being symmetric), the diagonal and 64
                       seeded random columns at energies 47 and 63 (729 and 800 open), none
                       open at energy 0; per energy the oracle's unitarity defect and the FP64
-                      noise floor -- max |dS| between the solve-based oracle and the same
-                      oracle with explicit inverses in the sector step
-                      (oracle.ldmd.sector_step_inverse), two equally valid FP64 orderings.
+                      noise floor -- the larger max |dS| between the solve-based oracle and
+                      the same oracle with its sector inverses formed another way: explicitly
+                      (LAPACK getri, oracle.ldmd.sector_step_inverse) or by blocked
+                      Gauss-Jordan with partial pivoting (oracle.ldmd.sector_step_gj), the
+                      device's algorithm class, or that together with W contracted in
+                      another summation order (oracle.contraction.contract_split), as the
+                      device changes both; all are equally valid FP64 orderings; and
+                      the matching step's own floor, the distance between the last
+                      variant's Y matched with solves and with explicit inverses
+                      (oracle.matching.get_smat(inverse=True)).
 
 Every fixture is written by the oracle (tests/../oracle) or by mpmath/sympy; nothing
 here reads the reference sources.
@@ -127,6 +134,46 @@ def _rotor_rotor_N800_part(w, prob, k: int, parts_dir):
     return S, S_inv
 
 
+def _rotor_rotor_N800_gj_part(w, prob, k: int, parts_dir):
+    """S of the blocked Gauss-Jordan ordering at energy index k (cached in parts_dir)."""
+    from oracle import ldmd
+    from oracle import pipeline as opipe
+
+    path = None if parts_dir is None else Path(parts_dir) / f"n800_energy{k}_gj.npz"
+    if path is not None and path.exists():
+        return np.load(path)["S_gj"]
+    E = w.energies[[k]]
+    Y_gj = opipe.propagate(prob, E, steps=(ldmd.sector_step_gj,))
+    S_gj = opipe.match(prob, Y_gj, E)[2][0]
+    if path is not None:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        np.savez(path, S_gj=S_gj)
+    return S_gj
+
+
+def _rotor_rotor_N800_alt_part(w, prob, k: int, parts_dir):
+    """S at energy index k with both of the solve's FP64 orderings changed at once:
+    W contracted with each angular sum in two halves (oracle.contraction.contract_split)
+    and the sector inverses by blocked Gauss-Jordan (cached in parts_dir)."""
+    from oracle import contraction as ocontract
+    from oracle import ldmd
+    from oracle import pipeline as opipe
+
+    path = None if parts_dir is None else Path(parts_dir) / f"n800_energy{k}_alt.npz"
+    if path is not None and path.exists():
+        d = np.load(path)
+        return d["S_alt"], d["S_alt_match_inv"]
+    E = w.energies[[k]]
+    Y = opipe.propagate(prob, E, steps=(ldmd.sector_step_gj,), contract=ocontract.contract_split)
+    S_alt = opipe.match(prob, Y, E)[2][0]
+    # the same Y matched with explicit inverses: the matching step's own floor
+    S_alt_match_inv = opipe.match(prob, Y, E, inverse=True)[2][0]
+    if path is not None:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        np.savez(path, S_alt=S_alt, S_alt_match_inv=S_alt_match_inv)
+    return S_alt, S_alt_match_inv
+
+
 def make_rotor_rotor_N800(parts_dir=None, only=None) -> None:
     """The N = 800 fixture, one energy at a time; with parts_dir each energy's S
     matrices are cached there (so separate processes can split the energies, and
@@ -140,14 +187,23 @@ def make_rotor_rotor_N800(parts_dir=None, only=None) -> None:
     _, _, prob = oracle_problem(w)
     todo = CONFIG2_ENERGIES if only is None else tuple(only)
     parts = {k: _rotor_rotor_N800_part(w, prob, k, parts_dir) for k in todo}
+    gj = {k: _rotor_rotor_N800_gj_part(w, prob, k, parts_dir) for k in todo}
+    alt = {k: _rotor_rotor_N800_alt_part(w, prob, k, parts_dir) for k in todo}
     if only is not None:
         return
     S = [parts[k][0] for k in CONFIG2_ENERGIES]
     S_inv = [parts[k][1] for k in CONFIG2_ENERGIES]
+    S_gj = [gj[k] for k in CONFIG2_ENERGIES]
+    dist = lambda a, b: float(np.max(np.abs(a - b))) if a.size else 0.0  # noqa: E731
+    floor_inv = np.array([dist(a, b) for a, b in zip(S, S_inv)])
+    floor_gj = np.array([dist(a, b) for a, b in zip(S, S_gj)])
+    floor_alt = np.array([dist(a, alt[k][0]) for a, k in zip(S, CONFIG2_ENERGIES)])
+    floor_match = np.array([dist(alt[k][0], alt[k][1]) for k in CONFIG2_ENERGIES])
     rng = np.random.default_rng(800)
     out = {"energy_index": np.array(CONFIG2_ENERGIES), "energies": w.energies[list(CONFIG2_ENERGIES)],
            "defect": np.array([omatch.unitarity_defect(s) if s.size else 0.0 for s in S]),
-           "floor": np.array([float(np.max(np.abs(a - b))) if a.size else 0.0 for a, b in zip(S, S_inv)]),
+           "floor_inverse": floor_inv, "floor_gj": floor_gj, "floor_alt": floor_alt,
+           "floor": np.maximum(np.maximum(floor_inv, floor_gj), floor_alt), "floor_match": floor_match,
            "n_open": np.array([s.shape[0] for s in S])}
     for i, s in enumerate(S):
         n = s.shape[0]

@@ -5,7 +5,8 @@ from __future__ import annotations
 import numpy as np
 
 from oracle.pipeline import Problem
-from tests.golden.fixtures import n800_tolerance, rotor_rotor_N800_reference  # noqa: F401
+from tests.golden.fixtures import (  # noqa: F401
+    n800_match_tolerance, n800_tolerance, n800_total_tolerance, rotor_rotor_N800_reference)
 from pyticc_amd import prepare_potential
 from pyticc_amd.matrix.centrifugal import get_Umat_BF, ladder_labels
 from pyticc_amd.scattering.hamiltonian import build_hamiltonian

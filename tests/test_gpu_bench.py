@@ -34,7 +34,12 @@ def test_headline_bench_line(gpu_device):
     r = d["roofline"]
     assert r["bound"] == "mfma" and r["unit"] == "TFLOP/s" and r["peak"] == 78.6
     assert r["achieved"] > 0 and 0 < r["frac"] < 1
-    assert abs(r["achieved"] - r["flop_per_launch"] / (r["avg_launch_us"] * 1e-6) / 1e12) < 1e-6 * r["achieved"]
+    # the rate is the timed launches' flop over the union of their intervals,
+    # which is at most their summed time and at least the longest launch
+    busy = r["busy_ms"] * 1e-3
+    assert abs(r["achieved"] - r["flop_per_launch"] * r["timed_launches"] / busy / 1e12) < 1e-6 * r["achieved"]
+    assert r["avg_launch_us"] * 1e-3 <= r["busy_ms"] <= r["summed_launch_ms"] * (1 + 1e-9)
+    assert r["mean_concurrent_launches"] >= 1 - 1e-9
     assert 0 < r["hbm_frac"] < 1
     assert 0 < d["sweep"]["frac_of_fp64_peak"] < 1
     assert d["parity"] is None  # a reduced run names no fixture comparison

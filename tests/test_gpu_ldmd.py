@@ -119,3 +119,20 @@ def test_zero_sectors_leave_Y_unchanged(gpu_device):
     empty = np.zeros((0, N, N))
     got = _device.to_host(propagate_logD(Y0, [0.1, 0.2, 0.3], 2.0, [], empty, empty, empty))
     np.testing.assert_array_equal(got, Y0)
+
+
+def test_default_two_groups_equal_one_group(gpu_device, monkeypatch):
+    """At Np >= 256 with 16 or more energies the sweep runs two energy groups on
+    their own streams by default: bit for bit the one-group sweep."""
+    rng = np.random.default_rng(31)
+    N, n_E, n_S = 300, 16, 5
+    Ws, Wm, We = random_W_batch(rng, n_S, N, scale=20.0)
+    hs = rng.uniform(0.01, 0.03, n_S)
+    E = rng.uniform(0.5, 2.0, n_E)
+    mu = 3.0
+    Y0 = oldmd.initialize_inelastic(Ws[0][None] - 2 * mu * E[:, None, None] * np.eye(N))
+    print(f"default groups: N={N} energies={n_E} sectors={n_S}", flush=True)
+    default = _device.to_host(propagate_logD(Y0, E, mu, hs, Ws, Wm, We))
+    monkeypatch.setenv("TICC_SWEEP_GROUPS", "1")
+    one = _device.to_host(propagate_logD(Y0, E, mu, hs, Ws, Wm, We))
+    np.testing.assert_array_equal(default, one)

@@ -193,3 +193,21 @@ def test_mixed_solve_rotor_rotor_N800_full_size_matches_fixture(gpu_device):
     for e, n, err, defect in report:
         assert err <= TOL_MIXED, (e, n, err)
         assert defect < 1e-9, (e, defect)
+
+
+def test_mixed_sweep_default_two_groups_equal_one_group(gpu_device, monkeypatch):
+    """The mixed sweep's default of two energy groups (Np >= 256, 16 or more
+    energies) gives the one-group mixed sweep bit for bit, fallbacks included."""
+    rng = np.random.default_rng(37)
+    N, n_E, n_S = 300, 16, 4
+    Ws, Wm, We = random_W_batch(rng, n_S, N, scale=20.0)
+    We[:-1] = Ws[1:]
+    hs = rng.uniform(0.01, 0.03, n_S)
+    E = rng.uniform(0.5, 2.0, n_E)
+    mu = 3.0
+    Y0 = oldmd.initialize_inelastic(Ws[0][None] - 2 * mu * E[:, None, None] * np.eye(N))
+    print(f"mixed default groups: N={N} energies={n_E} sectors={n_S}", flush=True)
+    default = _sweep_mixed(gpu_device, Y0, E, mu, hs, Ws, Wm, We, 2)
+    monkeypatch.setenv("TICC_SWEEP_GROUPS", "1")
+    one = _sweep_mixed(gpu_device, Y0, E, mu, hs, Ws, Wm, We, 2)
+    np.testing.assert_array_equal(default, one)

@@ -52,3 +52,19 @@ def test_w_signs():
     W = oc.w_full(2.0, 0.7, 1.5, np.array([0.0, 0.2]), U, V)
     expect = U / 4.0 + 3.0 * V + np.diag(3.0 * (np.array([0.0, 0.2]) - 0.7))
     np.testing.assert_allclose(W, expect)
+
+
+def test_split_summation_order_agrees():
+    """contract_split (each angular sum in two halves) is the same arithmetic in
+    another order: equal to rounding on random data, and bit for bit on small
+    integers, where every partial sum is exact."""
+    rng = np.random.default_rng(4)
+    idx = np.arange(6)
+    Br, Bi = rng.standard_normal((6, 41)), rng.standard_normal((6, 41))
+    V = rng.standard_normal((3, 41))
+    a = oc.contract([(idx, Br, Bi)], 6, 0.7, V)
+    b = oc.contract_split([(idx, Br, Bi)], 6, 0.7, V)
+    np.testing.assert_allclose(b, a, rtol=0, atol=1e-13 * np.abs(a).max())
+    Bn, Vn = rng.integers(-3, 4, (6, 41)).astype(float), rng.integers(-5, 6, (2, 41)).astype(float)
+    np.testing.assert_array_equal(oc.contract_split([(idx, Bn, None)], 6, 1.0, Vn),
+                                  oc.contract([(idx, Bn, None)], 6, 1.0, Vn))

@@ -122,3 +122,45 @@ def test_fourth_order_check_catches_planted_mistakes(monkeypatch, mistake):
     monkeypatch.setattr(ldmd, "correction_matrices", planted)
     with pytest.raises(AssertionError):
         test_coupled_constant_W_converges_at_fourth_order()
+
+
+@pytest.mark.parametrize("n", [1, 7, 32, 33, 96, 161])
+def test_gauss_jordan_inverse_matches_lapack(n):
+    """The oracle's blocked Gauss-Jordan inverse (the device's algorithm class)
+    against LAPACK's inverse, on random matrices, a permutation matrix and a
+    matrix with a zero diagonal (every column needs a row interchange)."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 0.1 * np.eye(n)
+    ref = np.linalg.inv(A)
+    np.testing.assert_allclose(ldmd.gauss_jordan_inverse(A), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    Pm = np.eye(n)[rng.permutation(n)]
+    np.testing.assert_array_equal(ldmd.gauss_jordan_inverse(Pm), Pm.T)
+    Z = rng.standard_normal((n, n))
+    np.fill_diagonal(Z, 0.0)
+    if n > 1:
+        np.testing.assert_allclose(ldmd.gauss_jordan_inverse(Z) @ Z, np.eye(n), rtol=0, atol=1e-11)
+
+
+def test_gauss_jordan_inverse_block_size_independent():
+    """Blocking changes only the rounding: nb = 1 (column by column), 8, 32 and a
+    block wider than the matrix give the same inverse to rounding."""
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((70, 70))
+    ref = ldmd.gauss_jordan_inverse(A, nb=1)
+    for nb in (8, 32, 128):
+        np.testing.assert_allclose(ldmd.gauss_jordan_inverse(A, nb=nb), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def test_sector_orderings_agree():
+    """The solve-based, explicit-inverse and Gauss-Jordan sector steps are FP64
+    orderings of the same arithmetic: one coupled sector agrees to rounding."""
+    rng = np.random.default_rng(11)
+    n, h = 24, 0.02
+    B = rng.standard_normal((n, n)) * 0.3
+    Wm = 0.5 * (B + B.T) + np.diag(rng.uniform(-2.0, 4.0, n))
+    Ws, We = Wm + 0.01, Wm - 0.01
+    ref, Qs, Qm, Qe = ldmd.correction_matrices(h, Ws, Wm, We)
+    Y = np.diag(rng.uniform(0.5, 2.0, n))
+    a = ldmd.sector_step(Y, h, ref, Qs, Qm, Qe)
+    for step in (ldmd.sector_step_inverse, ldmd.sector_step_gj):
+        np.testing.assert_allclose(step(Y, h, ref, Qs, Qm, Qe), a, rtol=0, atol=1e-11 * np.abs(a).max())

@@ -114,3 +114,21 @@ def test_partial_cross_section_detailed_balance():
     kk = np.array([1.1, 0.7])
     gg = np.array([2, 3])
     np.testing.assert_allclose(kk[:, None] ** 2 * gg[:, None] * sig, (kk[:, None] ** 2 * gg[:, None] * sig).T, rtol=1e-12)
+
+
+def test_matching_orderings_agree():
+    """The matching with its solves formed through explicit inverses (the ordering
+    whose distance sets the matching's noise floor) equals the solve-based S to
+    rounding, with closed channels eliminated."""
+    rng = np.random.default_rng(21)
+    N = 30
+    A = rng.standard_normal((2, N, N))
+    Y = 0.5 * (A + np.swapaxes(A, 1, 2)) + np.diag(rng.uniform(1.0, 5.0, N))
+    E_int = np.sort(rng.uniform(0.0, 2.0, N))
+    L = rng.integers(0, 8, N).astype(float)
+    E = np.array([0.7, 1.6])
+    a = om.get_smat(Y, 12.0, E, 3.0, E_int, L)
+    b = om.get_smat(Y, 12.0, E, 3.0, E_int, L, inverse=True)
+    for x, y in zip(a, b):
+        assert x.shape == y.shape and x.size
+        np.testing.assert_allclose(y, x, rtol=0, atol=1e-12)
