@@ -9,7 +9,11 @@ Pins (tests/test_oracle_ldmd.py):
   * constant *coupled* W: converges to the exact eigenbasis propagator with
     O(h^4) error (a dropped Q term or wrong Q_mid sign breaks the order);
   * batched sweep == sequential single-sector calls (test_logd.py:98-144);
-  * initial conditions (test_logd.py:34-43).
+  * initial conditions (test_logd.py:34-43);
+  * the other FP64 orderings of the sector step (explicit inverses, blocked
+    Gauss-Jordan) agree with the solve-based step to rounding, and the
+    Gauss-Jordan inverse equals LAPACK's on random, permutation and zero-diagonal
+    matrices at every block size.
 """
 
 from __future__ import annotations

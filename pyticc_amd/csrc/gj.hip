@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "ticc_common.h"
@@ -515,7 +516,10 @@ __device__ __forceinline__ void panel_gemm_body(const T* __restrict__ A, const T
         const int g = (int)__builtin_amdgcn_raw_buffer_load_b32(rG, live ? 4 * row : kOutOfRange, 0, 0);
         src[u][r] = live ? g : -1;
       }
-    typename E::acc acc[GCH][2];
+    // (the products are accumulated from zero and C added once at the end, as
+    // in the update: an accumulator started at C would take the NB products one
+    // at a time onto it)
+    typename E::acc cvals[GCH][2];
     T a[GCH][NB / 4];
 #pragma unroll
     for (int u = 0; u < GCH; ++u) {
@@ -528,19 +532,28 @@ __device__ __forceinline__ void panel_gemm_body(const T* __restrict__ A, const T
         const int s = src[u][r];
         const int base = s >= 0 ? (int)sizeof(T) * (s * ldC + colC + lr) : kOutOfRange;
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) acc[u][bj][r] = load1<T>(rC, base + 16 * (int)sizeof(T) * bj);
+        for (int bj = 0; bj < 2; ++bj) cvals[u][bj][r] = load1<T>(rC, base + 16 * (int)sizeof(T) * bj);
       }
     }
 #pragma unroll
     for (int u = 0; u < GCH; ++u) {
       const int row0 = (c0 + u) * RB + wave * 16;
+      typename E::acc acc[2];
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[bj][r] = T(0);
 #pragma unroll
       for (int kk = 0; kk < NB / 4; ++kk) {
         const int kq = 4 * kk + lq;
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj)
-          acc[u][bj] = E::mfma(a[u][kk], sB[kq][16 * bj + lr], acc[u][bj]);
+          acc[bj] = E::mfma(a[u][kk], sB[kq][16 * bj + lr], acc[bj]);
       }
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[bj][r] = cvals[u][bj][r] + acc[bj][r];
       if (ostride > 0) {
         T* Ob = Out + (size_t)b * ostride;
 #pragma unroll
@@ -548,7 +561,7 @@ __device__ __forceinline__ void panel_gemm_body(const T* __restrict__ A, const T
           const int row = row0 + E::drow(lq, r);
           if (row < Np) {
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj) Ob[(size_t)(16 * bj + lr) * Np + row] = acc[u][bj][r];
+            for (int bj = 0; bj < 2; ++bj) Ob[(size_t)(16 * bj + lr) * Np + row] = acc[bj][r];
           }
         }
       } else {
@@ -558,7 +571,7 @@ __device__ __forceinline__ void panel_gemm_body(const T* __restrict__ A, const T
           const int row = row0 + E::drow(lq, r);
           if (row < Np) {
 #pragma unroll
-            for (int bj = 0; bj < 2; ++bj) Ob[(size_t)row * NB + 16 * bj + lr] = acc[u][bj][r];
+            for (int bj = 0; bj < 2; ++bj) Ob[(size_t)row * NB + 16 * bj + lr] = acc[bj][r];
           }
         }
       }
@@ -623,9 +636,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // is 16 consecutive doubles.  A workgroup of NBANDS x 4 waves (32 x 32 each)
 // walks NBANDS adjacent 64-row bands across a group of 64-column tiles, sharing
 // each staged B tile between the bands: the bands' A fragments stay in
-// registers, and the next tile's C rows (into the idle accumulator set) and B
-// rows (into the idle LDS buffer) are loaded while the current tile's MFMAs
-// and stores run.
+// registers; a tile's C rows are loaded before its MFMAs (which accumulate the
+// products from zero; C is added once at the end) and the next tile's B rows
+// (into the idle LDS buffer) while the current tile's MFMAs and stores run.
 template <int KB, int NBANDS, bool ONE, typename T>
 __device__ __forceinline__ void gj_update_body(const T* __restrict__ X, T* __restrict__ Out,
                                                long long ostride, int ldo, int ocol0, int Np, int k0,
@@ -676,8 +689,8 @@ __device__ __forceinline__ void gj_update_body(const T* __restrict__ X, T* __res
     return (j0 >= k0 && j0 < k0 + KB) ? 1 : 0;
   };
   // live = false: every load takes the out-of-range offset (reads 0, no
-  // memory access).  The prefetches are issued on every path so the compiler's
-  // wait counts before the current tile's MFMAs need not cover them.
+  // memory access).  The B prefetch is issued on every path so the compiler's
+  // wait counts before the current tile's MFMAs need not cover it.
   auto load_c = [&](int J, bool live, typename E::acc(&c)[2][2]) {
     const int j0 = J * 64 + jw;
     const bool upd = live && kind(J) == 0;
@@ -707,10 +720,24 @@ __device__ __forceinline__ void gj_update_body(const T* __restrict__ X, T* __res
       *reinterpret_cast<typename E::vec2*>(&sB[buf][k][c2 ^ ((k & 1) << 4)]) = bst[h];
     }
   };
-  auto compute_store = [&](int J, int buf, typename E::acc(&acc)[2][2]) {
+  // The products are accumulated from zero and the tile's C values added once
+  // at the end: an accumulator that started at C would take the KB products
+  // one at a time onto the much larger C (the MFMA adds its four products
+  // sequentially), losing in a biased way the parts of small same-signed
+  // products below C's last bit (the inverse of I - h^2 T/6 came out with a
+  // mean diagonal error of -6 ulp).  C's loads are issued before the tile's
+  // MFMAs, which hide their latency.
+  auto compute_store = [&](int J, int buf, const typename E::acc(&cv)[2][2]) {
     const int kd = kind(J);
     const int j0 = J * 64 + jw;
     if (kd == 0) {
+      typename E::acc acc[2][2];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[bi][bj][r] = T(0);
 #pragma unroll
       for (int kk = 0; kk < KK; ++kk) {
         const int kq = 4 * kk + lq;
@@ -731,8 +758,9 @@ __device__ __forceinline__ void gj_update_body(const T* __restrict__ X, T* __res
           T* orow = Ob + (size_t)(i0 + 16 * bi + E::drow(lq, r)) * ldo + j0 + lr;
 #pragma unroll
           for (int bj = 0; bj < 2; ++bj) {
-            if constexpr (TICC_UPD_STORE_NT) __builtin_nontemporal_store(acc[bi][bj][r], orow + 16 * bj);
-            else orow[16 * bj] = acc[bi][bj][r];
+            const T v = cv[bi][bj][r] + acc[bi][bj][r];
+            if constexpr (TICC_UPD_STORE_NT) __builtin_nontemporal_store(v, orow + 16 * bj);
+            else orow[16 * bj] = v;
           }
         }
     } else if (kd == 1) {  // coefficient columns: Out[i][j0 + c] = AT[j0 - k0 + c][i]
@@ -756,28 +784,28 @@ __device__ __forceinline__ void gj_update_body(const T* __restrict__ X, T* __res
     }
   };
 
-  typename E::acc accA[2][2], accB[2][2];
+  typename E::acc cv[2][2];
   load_b(jt0, true);
-  load_c(jt0, true, accA);
   store_b(0);
   __syncthreads();
   if constexpr (ONE) {  // a single column tile: nothing to prefetch, one B buffer
-    compute_store(jt0, 0, accA);
+    load_c(jt0, true, cv);
+    compute_store(jt0, 0, cv);
   } else {
   for (int J = jt0;; J += 2) {
-    // even tile: accumulators A, LDS buffer 0; prefetch into B / buffer 1
+    // even tile: LDS buffer 0; the next tile's B rows are loaded into registers
     const bool more1 = J + 1 < jt1;
+    load_c(J, true, cv);
     load_b(J + 1, more1);
-    load_c(J + 1, more1, accB);
-    compute_store(J, 0, accA);
+    compute_store(J, 0, cv);
     if (!more1) break;
     store_b(1);
     __syncthreads();
-    // odd tile: accumulators B, LDS buffer 1; prefetch into A / buffer 0
+    // odd tile: LDS buffer 1
     const bool more2 = J + 2 < jt1;
+    load_c(J + 1, true, cv);
     load_b(J + 2, more2);
-    load_c(J + 2, more2, accA);
-    compute_store(J + 1, 1, accB);
+    compute_store(J + 1, 1, cv);
     if (!more2) break;
     store_b(0);
     __syncthreads();
@@ -1169,6 +1197,23 @@ int gj_invert_t(T* bufA, T* bufB, int batch, int Np, T** result, int* perm, int*
   T* buf[2] = {bufA, bufB};
   const int steps = Np / NB;
   const int pairs = steps / 2;
+  // TICC_GJ_SINGLE_STEPS=1: every block step on its own -- its panel, then a
+  // rank-NB update of the whole matrix -- with no pairing of steps and no
+  // look-ahead.  The same elimination and pivots in another FP64 ordering (the
+  // pair composition and the MFMA accumulation order change); slower, used to
+  // measure the device's own rounding spread.
+  if (const char* env = getenv("TICC_GJ_SINGLE_STEPS")) {
+    if (env[0] == '1') {
+      T* cur = bufA;
+      T* nxt = bufB;
+      for (int s = 0; s < steps; ++s) {
+        TICC_RETURN_IF(single_step(cur, nxt, batch, Np, s * NB, c, perm, pinv, mask, stream));
+        std::swap(cur, nxt);
+      }
+      *result = cur;
+      return TICC_OK;
+    }
+  }
   // Profile slots of this inversion's fused launches (one per pair but the last,
   // plus the last pair's when a trailing single step follows), reserved under
   // the lock so that inversions on concurrent lanes never share a slot.
