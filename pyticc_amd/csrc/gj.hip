@@ -144,6 +144,14 @@ __device__ __forceinline__ float readlane_t<float>(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// A wave's pivot candidate in the panel's column step (32 bytes).
+template <typename T>
+struct alignas(16) PanelRec {
+  unsigned long long key;
+  int pos, row;
+  T pinv, pnext;
+};
+
 // Panel: one workgroup per matrix; thread t owns rows t + THREADS q
 // (q < RPT, Np <= THREADS RPT) with their NB panel values in registers and
 // its rows' current positions.  Per column: each wave reduces its best
@@ -161,13 +169,10 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
                                               int prev_k0, int* __restrict__ src2, int* __restrict__ brow, int b) {
   typedef Elem<T> E;
   constexpr int NW = THREADS / 64;
-  __shared__ T s_vals[2][NW][RPT][NB];
+  static_assert((THREADS & (THREADS - 1)) == 0, "the panel's thread count is a power of two");
+  __shared__ PanelRec<T> s_rec[2][NW];
+  __shared__ T s_prow[2][NB];
   __shared__ T s_inv[NB];
-  __shared__ unsigned long long s_key[2][NW];
-  __shared__ int s_pos[2][NW];
-  __shared__ int s_row[2][NW];
-  __shared__ T s_pinv[2][NW];
-  __shared__ T s_pnext[2][NW];
 
   const int t = threadIdx.x;
   const int lane = t & 63;
@@ -264,25 +269,26 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
         wl = __ffsll((long long)__ballot(top && cv == wv && cp == wp)) - 1;
       }
     }
-    // The wave's winning lane publishes its row (each of a thread's rows has its
-    // own LDS slot, so the write needs no register select), its key, position,
-    // row, the reciprocal of its pivot value and its value in the next column.
+    // The wave's winning lane publishes its candidate's record: key, position,
+    // row, the reciprocal of its pivot value and its value in the next column
+    // (two 16-byte writes; the row itself is written once the global winner is
+    // known).
     if (lane == wl) {
-#pragma unroll
-      for (int q = 0; q < RPT; ++q) {
-        if (cq == q) {
-#pragma unroll
-          for (int l = 0; l < NB; ++l) s_vals[buf][wave][q][l] = v[q][l];
-        }
-      }
-      s_key[buf][wave] = key;
-      s_pos[buf][wave] = cp;
-      s_row[buf][wave] = t + THREADS * cq;
-      s_pinv[buf][wave] = rinv;
-      s_pnext[buf][wave] = pnx;
+      PanelRec<T> rec;
+      rec.key = key;
+      rec.pos = cp;
+      rec.row = t + THREADS * cq;
+      rec.pinv = rinv;
+      rec.pnext = pnx;
+      s_rec[buf][wave] = rec;
     } else if (wl < 0 && lane == 0) {
-      s_key[buf][wave] = 0ull;
-      s_pos[buf][wave] = 0x7fffffff;
+      PanelRec<T> rec;
+      rec.key = 0ull;
+      rec.pos = 0x7fffffff;
+      rec.row = 0;
+      rec.pinv = T(0);
+      rec.pnext = T(0);
+      s_rec[buf][wave] = rec;
     }
     __syncthreads();
     // Global winner: larger key, smaller position on ties.  Lane l reads wave
@@ -290,14 +296,15 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     // all records, and log2(NW) DPP row rotations leave the winner in every
     // lane; its row, reciprocal and next-column value are then read from the
     // lane that loaded its record.  (A wave without a candidate has key 0 and
-    // stale other fields, and never wins: some wave always has a candidate.)
+    // never wins: some wave always has a candidate.)
     const int wr = lane & (NW - 1);
-    unsigned long long kb = s_key[buf][wr];
-    int pb = s_pos[buf][wr];
+    const PanelRec<T> rr = s_rec[buf][wr];
+    unsigned long long kb = rr.key;
+    int pb = rr.pos;
     int wb = wr;
-    const int rrow = s_row[buf][wr];
-    const T rpinv = s_pinv[buf][wr];
-    const T rpn = s_pnext[buf][wr];
+    const int rrow = rr.row;
+    const T rpinv = rr.pinv;
+    const T rpn = rr.pnext;
     panel_winner_step<0x121>(kb, pb, wb);  // row_ror:1
     if constexpr (NW > 2) panel_winner_step<0x122>(kb, pb, wb);  // row_ror:2
     if constexpr (NW > 4) panel_winner_step<0x124>(kb, pb, wb);  // row_ror:4
@@ -306,7 +313,18 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     const int pr = __builtin_amdgcn_readlane(rrow, bw);
     const T inv = readlane_t<T>(rpinv, bw);
     const T pn = readlane_t<T>(rpn, bw);
-    const T* prow = s_vals[buf][bw][pr / THREADS];
+    // The pivot row's owner writes it for every thread; the second barrier
+    // below, after the next column's update, publishes it.
+    if (t == (pr & (THREADS - 1))) {
+      const int pq = pr / THREADS;
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        if (q == pq) {
+#pragma unroll
+          for (int l = 0; l < NB; ++l) s_prow[buf][l] = v[q][l];
+        }
+      }
+    }
     // Gauss-Jordan step with the pivot row's scaling deferred: every other row
     // r takes a_rl -= (a_rj / p) p_l, a_rj = -a_rj / p; the pivot row keeps its
     // values with a_rj = 1.  A row that was a pivot row stays p times its true
@@ -327,6 +345,8 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     // the window; pn is the pivot row's value there either way)
 #pragma unroll
     for (int q = 0; q < RPT; ++q) v[q][u + 1] = E::fma(-g[q], pn, v[q][u + 1]);
+    __syncthreads();
+    const T* prow = s_prow[buf];
 #pragma unroll
     for (int l = 0; l < NB; ++l) {
       if (l == u || l == u + 1) continue;
