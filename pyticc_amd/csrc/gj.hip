@@ -166,7 +166,9 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
                                               int* __restrict__ piv, int* __restrict__ perm, int* __restrict__ pinv,
                                               T* __restrict__ PoutT, long long tstride,
                                               const int* __restrict__ prev_src, const int* __restrict__ prev_piv,
-                                              int prev_k0, int* __restrict__ src2, int* __restrict__ brow, int b) {
+                                              int prev_k0, int* __restrict__ src2, int* __restrict__ brow, int b,
+                                              int* lsrc = nullptr, int* lpiv = nullptr,
+                                              const int* lprev_src = nullptr, const int* lprev_piv = nullptr) {
   typedef Elem<T> E;
   constexpr int NW = THREADS / 64;
   static_assert((THREADS & (THREADS - 1)) == 0, "the panel's thread count is a power of two");
@@ -395,13 +397,15 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     src[(size_t)b * Np + p] = r;
     const bool blk = (p >= k0 && p < k0 + NB);
     if (blk) piv[b * NB + (p - k0)] = r;
+    if (lsrc) lsrc[p] = r;
+    if (blk && lpiv) lpiv[p - k0] = r;
     if (PoutT) {
 #pragma unroll
       for (int l = 0; l < NB; ++l) PoutT[(size_t)b * tstride + (size_t)l * Np + p] = v[q][l];
     }
     if (src2) {  // row maps of the update that follows this panel
       if (prev_src) {  // second step of a pair: compose with the first step's maps
-        const int s1r = prev_src[(size_t)b * Np + r];
+        const int s1r = lprev_src ? lprev_src[r] : prev_src[(size_t)b * Np + r];
         src2[(size_t)b * Np + p] = (blk || (r >= prev_k0 && r < prev_k0 + NB)) ? -1 : s1r;
         if (blk) brow[b * 2 * NB + NB + (p - k0)] = s1r;
       } else {
@@ -411,7 +415,7 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     }
     pold[q] = (k0 == 0) ? r : perm[(size_t)b * Np + r];
   }
-  if (src2 && prev_src && t < NB) brow[b * 2 * NB + t] = prev_piv[b * NB + t];
+  if (src2 && prev_src && t < NB) brow[b * 2 * NB + t] = lprev_piv ? lprev_piv[t] : prev_piv[b * NB + t];
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
@@ -493,16 +497,16 @@ __device__ __forceinline__ T load1_cpol(__amdgpu_buffer_rsrc_t r, int off) {
 //   Out[i][0:NB] = mask(i) C[gC(i)][colC : colC+NB] + A[i][0:NB] . B[gB(r)][colB : colB+NB],
 //   mask(i) = 0 for i in [mlo, mlo+NB), 1 otherwise.
 // Out is row-major (Np x NB) or, with `ostride` > 0, transposed (NB rows of
-// length Np, matrix stride ostride).  Wave w takes the 16-row blocks
-// w, w + THREADS/64, ...; the row-source indices and then the C and A
-// operands of GCH blocks are loaded together, so the gathers' latency is paid
-// twice per GCH blocks instead of twice per block.  sB stages the NB gathered
-// B rows.
+// length Np, matrix stride ostride).  The row maps gC (Np entries) and gB
+// (NB entries) are the panel's, in this workgroup's LDS.  Wave w takes the
+// 16-row blocks w, w + THREADS/64, ...; the C and A operands of GCH blocks are
+// loaded together, so the gathers' latency is paid once per GCH blocks instead
+// of once per block.  sB stages the NB gathered B rows.
 template <int THREADS, int MAXROWS, typename T>
 __device__ __forceinline__ void panel_gemm_body(const T* __restrict__ A, const T* __restrict__ C,
-                                                long long cb, int ldC, int colC, const int* __restrict__ gC,
+                                                long long cb, int ldC, int colC, const int* gC,
                                                 const T* __restrict__ Bm, long long bb, int ldB, int colB,
-                                                const int* __restrict__ gB, int mlo, int Np,
+                                                const int* gB, int mlo, int Np,
                                                 T* __restrict__ Out, long long ostride, int b,
                                                 T (*sB)[NB + 1]) {
   typedef Elem<T> E;
@@ -515,12 +519,10 @@ __device__ __forceinline__ void panel_gemm_body(const T* __restrict__ A, const T
   const T* Cb = C + (size_t)b * cb;
   const auto rC = make_rsrc(Cb, (unsigned)(sizeof(T) * Np * ldC));
   const auto rA = make_rsrc(A + (size_t)b * Np * NB, (unsigned)(sizeof(T) * Np * NB));
-  const auto rG = make_rsrc(gC + (size_t)b * Np, (unsigned)(sizeof(int) * Np));
   const T* Bb = Bm + (size_t)b * bb;
-  const int* gBb = gB + b * NB;
   for (int e = t; e < NB * NB; e += THREADS) {
     const int r = e / NB, cc = e % NB;
-    sB[r][cc] = Bb[(size_t)gBb[r] * ldB + colB + cc];
+    sB[r][cc] = Bb[(size_t)gB[r] * ldB + colB + cc];
   }
   __syncthreads();
   for (int c0 = 0; c0 < MAXIT; c0 += GCH) {
@@ -533,8 +535,7 @@ __device__ __forceinline__ void panel_gemm_body(const T* __restrict__ A, const T
       for (int r = 0; r < 4; ++r) {
         const int row = (c0 + u) * RB + wave * 16 + E::drow(lq, r);
         const bool live = row < Np && !(row >= mlo && row < mlo + NB);
-        const int g = (int)__builtin_amdgcn_raw_buffer_load_b32(rG, live ? 4 * row : kOutOfRange, 0, 0);
-        src[u][r] = live ? g : -1;
+        src[u][r] = live ? gC[row] : -1;
       }
     // (the products are accumulated from zero and C added once at the end, as
     // in the update: an accumulator started at C would take the NB products one
@@ -620,6 +621,10 @@ __device__ __forceinline__ void gj_pair_chain_body(const PairJob<T>& P, int b) {
   // row bound of the panel GEMMs: 1024 up to two rows per thread, else MAX_NP
   constexpr int GR = RPT <= 2 ? 1024 : MAX_NP;
   __shared__ T sG[NB][NB + 1];
+  // each panel's row map and pivot rows, also kept here for the GEMM after it
+  // and the second panel (the global copies serve the update)
+  __shared__ int s_map[2][GR];
+  __shared__ int s_pv[2][NB];
   const int Np = P.Np, k0 = P.k0, k1 = P.k0 + NB;
   const long long pn = (long long)Np * NB;
   const long long at_stride = 2LL * NB * Np;
@@ -629,15 +634,15 @@ __device__ __forceinline__ void gj_pair_chain_body(const PairJob<T>& P, int b) {
     return;
   }
   gj_panel_body<512, RPT, T>(P.Xc, P.xs, P.ldx, P.xcol, Np, k0, P.P1, P.s1, P.p1, P.perm, P.pinv, nullptr, 0, nullptr,
-                          nullptr, 0, nullptr, nullptr, b);
+                          nullptr, 0, nullptr, nullptr, b, s_map[0], s_pv[0]);
   __syncthreads();
-  panel_gemm_body<512, GR, T>(P.P1, P.Xc, P.xs, P.ldx, P.xcol + NB, P.s1, P.Xc, P.xs, P.ldx, P.xcol + NB, P.p1, k0, Np, P.Z,
-                       0LL, b, sG);
+  panel_gemm_body<512, GR, T>(P.P1, P.Xc, P.xs, P.ldx, P.xcol + NB, s_map[0], P.Xc, P.xs, P.ldx, P.xcol + NB, s_pv[0], k0,
+                              Np, P.Z, 0LL, b, sG);
   __syncthreads();
   gj_panel_body<512, RPT, T>(P.Z, pn, NB, 0, Np, k1, P.P2, P.s2, P.p2, P.perm, P.pinv, P.AT + (size_t)NB * Np, at_stride,
-                          P.s1, P.p1, k0, P.src2, P.brow, b);
+                          P.s1, P.p1, k0, P.src2, P.brow, b, s_map[1], s_pv[1], s_map[0], s_pv[0]);
   __syncthreads();
-  panel_gemm_body<512, GR, T>(P.P2, P.P1, pn, NB, 0, P.s2, P.P1, pn, NB, 0, P.p2, k1, Np, P.AT, at_stride, b, sG);
+  panel_gemm_body<512, GR, T>(P.P2, P.P1, pn, NB, 0, s_map[1], P.P1, pn, NB, 0, s_pv[1], k1, Np, P.AT, at_stride, b, sG);
 }
 
 template <int RPT, typename T>
