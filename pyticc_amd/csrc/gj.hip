@@ -156,10 +156,14 @@ struct alignas(16) PanelRec {
 // (q < RPT, Np <= THREADS RPT) with their NB panel values in registers and
 // its rows' current positions.  Per column: each wave reduces its best
 // candidate (largest |a|, first position on ties) and its winning lane
-// publishes the candidate row's values in a double-buffered LDS slot; after
-// the column's single barrier every thread picks the global winner, applies
-// the row swap to its own position records and performs the in-place
-// Gauss-Jordan update, streaming the pivot row from LDS.
+// publishes a 32-byte record of it in a double-buffered LDS slot; after the
+// column's first barrier every thread picks the global winner, applies the
+// row swap to its own position records and updates the next column, while the
+// winner's owner writes its row to LDS; after the second barrier every thread
+// performs the rest of the in-place Gauss-Jordan update, streaming the pivot
+// row from LDS.  Optionally the panel also leaves its row map and pivot rows
+// in LDS (lsrc, lpiv) and reads the previous panel's there (lprev_src,
+// lprev_piv) instead of the global copies.
 template <int THREADS, int RPT, typename T>
 __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long bstride, int ld, int col0,
                                               int Np, int k0, T* __restrict__ Pout, int* __restrict__ src,
