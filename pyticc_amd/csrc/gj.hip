@@ -300,22 +300,40 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     // Global winner: larger key, smaller position on ties.  Lane l reads wave
     // (l mod NW)'s record, so every NW consecutive lanes of a 16-lane row hold
     // all records, and log2(NW) DPP row rotations leave the winner in every
-    // lane; its row, reciprocal and next-column value are then read from the
-    // lane that loaded its record.  (A wave without a candidate has key 0 and
-    // never wins: some wave always has a candidate.)
+    // lane; its row, reciprocal and next-column value are then read from lane
+    // bw, which loaded wave bw's record.  (A wave without a candidate has key 0
+    // and never wins: some wave always has a candidate.)
     const int wr = lane & (NW - 1);
     const PanelRec<T> rr = s_rec[buf][wr];
-    unsigned long long kb = rr.key;
-    int pb = rr.pos;
-    int wb = wr;
     const int rrow = rr.row;
     const T rpinv = rr.pinv;
     const T rpn = rr.pnext;
-    panel_winner_step<0x121>(kb, pb, wb);  // row_ror:1
-    if constexpr (NW > 2) panel_winner_step<0x122>(kb, pb, wb);  // row_ror:2
-    if constexpr (NW > 4) panel_winner_step<0x124>(kb, pb, wb);  // row_ror:4
-    const int bw = __builtin_amdgcn_readfirstlane(wb);
-    const int bp = __builtin_amdgcn_readfirstlane(pb);
+    // First the top 32 bits of the keys with their lowest log2(NW) bits
+    // replaced by the wave (smaller wave ranks higher): when only one wave's key
+    // reaches the maximum there, that wave's full key is larger than every
+    // other and it is the winner; otherwise the exact (key, position) reduction
+    // decides.
+    const unsigned wkhi = (unsigned)(rr.key >> 32) & ~(unsigned)(NW - 1);
+    unsigned cm = wkhi | (unsigned)(NW - 1 - wr);
+    cm = max(cm, (unsigned)__builtin_amdgcn_update_dpp(0, (int)cm, 0x121, 0xF, 0xF, false));  // row_ror:1
+    if constexpr (NW > 2) cm = max(cm, (unsigned)__builtin_amdgcn_update_dpp(0, (int)cm, 0x122, 0xF, 0xF, false));
+    if constexpr (NW > 4) cm = max(cm, (unsigned)__builtin_amdgcn_update_dpp(0, (int)cm, 0x124, 0xF, 0xF, false));
+    const unsigned cmax = (unsigned)__builtin_amdgcn_readfirstlane((int)cm);
+    const unsigned long long level = __ballot(lane < NW && wkhi == (cmax & ~(unsigned)(NW - 1)));
+    int bw, bp;
+    if (__popcll(level) == 1) {
+      bw = NW - 1 - (int)(cmax & (unsigned)(NW - 1));
+      bp = __builtin_amdgcn_readlane(rr.pos, bw);
+    } else {
+      unsigned long long kb = rr.key;
+      int pb = rr.pos;
+      int wb = wr;
+      panel_winner_step<0x121>(kb, pb, wb);  // row_ror:1
+      if constexpr (NW > 2) panel_winner_step<0x122>(kb, pb, wb);  // row_ror:2
+      if constexpr (NW > 4) panel_winner_step<0x124>(kb, pb, wb);  // row_ror:4
+      bw = __builtin_amdgcn_readfirstlane(wb);
+      bp = __builtin_amdgcn_readfirstlane(pb);
+    }
     const int pr = __builtin_amdgcn_readlane(rrow, bw);
     const T inv = readlane_t<T>(rpinv, bw);
     const T pn = readlane_t<T>(rpn, bw);
