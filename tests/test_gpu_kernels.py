@@ -68,6 +68,21 @@ def test_invert_singular_zero_column_gives_nonfinite_not_a_fault(gpu_device, N):
     np.testing.assert_array_equal(inv[2], _device.to_host(alone)[0])
 
 
+@pytest.mark.parametrize("n", [64, 512, 1024])
+def test_invert_hadamard_ties_across_waves(gpu_device, n):
+    """Sylvester Hadamard matrices: every entry is +-1, so every column's pivot
+    candidates tie in magnitude across all waves (the winner selection's exact
+    path), and the elimination stays in dyadic values, so the inverse H^T / n
+    comes out exactly (as the oracle's blocked Gauss-Jordan gives it).  A
+    winner of smaller magnitude, e.g. an eliminated zero, would show."""
+    H = np.array([[1.0]])
+    while H.shape[0] < n:
+        H = np.block([[H, H], [H, -H]])
+    d = _device.to_device(H[None], gpu_device)
+    _backend.call("ticc_invert_batched", 1, n, d.data_ptr(), _stream(gpu_device))
+    np.testing.assert_array_equal(_device.to_host(d)[0], H.T / n)
+
+
 def test_invert_requires_row_pivoting(gpu_device):
     # zero leading diagonal: Gauss-Jordan without pivoting would divide by zero
     A = np.array([[[0.0, 1.0, 2.0], [1.0, 0.0, 3.0], [4.0, 5.0, 0.0]]])
