@@ -151,6 +151,17 @@ def test_gauss_jordan_inverse_block_size_independent():
         np.testing.assert_allclose(ldmd.gauss_jordan_inverse(A, nb=nb), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("n", [2, 64, 256])
+def test_gauss_jordan_inverse_hadamard_exact(n):
+    """Sylvester Hadamard matrices (every pivot column tied in magnitude): the
+    Gauss-Jordan inverse is exactly H^T / n at every block size."""
+    H = np.array([[1.0]])
+    while H.shape[0] < n:
+        H = np.block([[H, H], [H, -H]])
+    for nb in (1, 32):
+        np.testing.assert_array_equal(ldmd.gauss_jordan_inverse(H, nb=nb), H.T / n)
+
+
 def test_sector_orderings_agree():
     """The solve-based, explicit-inverse and Gauss-Jordan sector steps are FP64
     orderings of the same arithmetic: one coupled sector agrees to rounding."""
