@@ -798,6 +798,10 @@ __device__ __forceinline__ void gj_update_body(const T* __restrict__ X, T* __res
           for (int bj = 0; bj < 2; ++bj)
             acc[bi][bj] = E::mfma(a[bi][kk], bv[bj], acc[bi][bj]);
       }
+      // Stores stream past the caches, except the one-tile look-ahead's, which
+      // the next panel chain reads right away.  (Keeping a pair update's tile
+      // that the next look-ahead reads cached as well was slower.)
+      const bool keep = ONE;
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -806,7 +810,7 @@ __device__ __forceinline__ void gj_update_body(const T* __restrict__ X, T* __res
 #pragma unroll
           for (int bj = 0; bj < 2; ++bj) {
             const T v = cv[bi][bj][r] + acc[bi][bj][r];
-            if constexpr (TICC_UPD_STORE_NT) __builtin_nontemporal_store(v, orow + 16 * bj);
+            if (TICC_UPD_STORE_NT && !keep) __builtin_nontemporal_store(v, orow + 16 * bj);
             else orow[16 * bj] = v;
           }
         }
