@@ -255,8 +255,11 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x124, 0xF, 0xF, false));  // row_ror:4
     m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x122, 0xF, 0xF, false));  // row_ror:2
     m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x121, 0xF, 0xF, false));  // row_ror:1
-    const unsigned M = max(max((unsigned)__builtin_amdgcn_readlane((int)m, 0), (unsigned)__builtin_amdgcn_readlane((int)m, 16)),
-                           max((unsigned)__builtin_amdgcn_readlane((int)m, 32), (unsigned)__builtin_amdgcn_readlane((int)m, 48)));
+    // the four rows' maxima into lane 63: row_bcast:15 into rows 1 and 3, then
+    // row_bcast:31 into rows 2 and 3 (lanes of the other rows take 0)
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x142, 0xA, 0xF, false));
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x143, 0xC, 0xF, false));
+    const unsigned M = (unsigned)__builtin_amdgcn_readlane((int)m, 63);
     // A lane has a candidate whenever cq >= 0, even when |a| is zero or
     // subnormal (its key then lies below 2^32 and khi is 0): a column of exact
     // zeros below the diagonal still yields a pivot (value 0, so the inverse is
