@@ -145,8 +145,16 @@ __device__ __forceinline__ float readlane_t<float>(float v, int lane) {
 }
 
 // A wave's pivot candidate in the panel's column step (32 bytes).
-template <typename T>
+// The one-barrier column schedule also carries the candidate's multiplier of
+// the previous pivot (gprev).
+template <typename T, bool GPREV>
 struct alignas(16) PanelRec {
+  unsigned long long key;
+  int pos, row;
+  T pinv, pnext, gprev;
+};
+template <typename T>
+struct alignas(16) PanelRec<T, false> {
   unsigned long long key;
   int pos, row;
   T pinv, pnext;
@@ -156,12 +164,14 @@ struct alignas(16) PanelRec {
 // (q < RPT, Np <= THREADS RPT) with their NB panel values in registers and
 // its rows' current positions.  Per column: each wave reduces its best
 // candidate (largest |a|, first position on ties) and its winning lane
-// publishes a 32-byte record of it in a double-buffered LDS slot; after the
-// column's first barrier every thread picks the global winner, applies the
-// row swap to its own position records and updates the next column, while the
-// winner's owner writes its row to LDS; after the second barrier every thread
-// performs the rest of the in-place Gauss-Jordan update, streaming the pivot
-// row from LDS.  Optionally the panel also leaves its row map and pivot rows
+// publishes a record of it in a double-buffered LDS slot; the column's
+// barrier publishes these records and the previous column's pivot row.  After
+// it every thread performs the previous pivot's in-place Gauss-Jordan update of
+// the other columns, streaming that pivot row from LDS, picks the global
+// winner, applies the row swap to its own position records and updates the
+// next column, while the winner's owner writes its row to LDS.  (With three
+// rows per thread a second barrier per column publishes the pivot row and the
+// other columns take the current pivot at once.)  Optionally the panel also leaves its row map and pivot rows
 // in LDS (lsrc, lpiv) and reads the previous panel's there (lprev_src,
 // lprev_piv) instead of the global copies.
 template <int THREADS, int RPT, typename T>
@@ -176,7 +186,8 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
   typedef Elem<T> E;
   constexpr int NW = THREADS / 64;
   static_assert((THREADS & (THREADS - 1)) == 0, "the panel's thread count is a power of two");
-  __shared__ PanelRec<T> s_rec[2][NW];
+  constexpr bool ONE_BARRIER = RPT <= 2;  // (the column schedule, below)
+  __shared__ PanelRec<T, ONE_BARRIER> s_rec[2][NW];
   __shared__ T s_prow[2][NB];
   __shared__ T s_inv[NB];
 
@@ -210,7 +221,52 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
   // window: v[q][l] holds panel column (j0 + l) % NB, so every iteration's
   // code is the same and the kernel stays small enough for the instruction
   // cache (a fully unrolled 32-column loop does not fit and runs from L2).
+  //
+  // Up to two rows per thread, one barrier per column: the update of the
+  // other columns by pivot j - 1 runs after column j's barrier (which
+  // publishes pivot row j - 1 with the candidate records).  A candidate's value
+  // in column j + 1 is then current through pivot j - 2 only, so its record
+  // also carries the row's pivot-(j - 1) multiplier, and the winner's value is
+  // brought through pivot j - 1 by the same FMA the row's own update applies:
+  // every value is formed by the same operations as with a second barrier per
+  // column.  (With three rows per thread the registers the deferred update
+  // keeps live make the panel spill: 8 x 1100 inversions took 9.0 against
+  // 7.0 ms.  There a second barrier per column publishes the pivot row and the
+  // other columns take it at once.)
   constexpr int PU = 4;
+  T g[RPT];     // the multipliers of the previous column's pivot
+  int ppr = -1;  // the previous column's pivot row
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) g[q] = T(0);
+  // Pivot (j - 1)'s update of the other columns at column step u of the
+  // window: its column is u - 1, or NB - 1 after the window's rotation at
+  // u = 0, where its pivot row in LDS still has the previous window's
+  // coordinates (index + PU); column u took it already, as the next column.
+  auto prev_update = [&](int u, const T* prow) {
+    const int lp = (u == 0) ? NB - 1 : u - 1;
+    const int sh = (u == 0) ? PU : 0;
+#pragma unroll
+    for (int l = 0; l < NB; ++l) {
+      if (l == u || l == lp) continue;
+      const T pl = prow[(l + sh) % NB];
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) v[q][l] = E::fma(-g[q], pl, v[q][l]);
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) v[q][lp] = (t + THREADS * q == ppr) ? T(1) : -g[q];
+  };
+  // The same update by the current column's pivot (two-barrier schedule).
+  auto cur_update = [&](int u, const T* prow, int pr) {
+#pragma unroll
+    for (int l = 0; l < NB; ++l) {
+      if (l == u || l == u + 1) continue;
+      const T pl = prow[l];
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) v[q][l] = E::fma(-g[q], pl, v[q][l]);
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) v[q][u] = (t + THREADS * q == pr) ? T(1) : -g[q];
+  };
   for (int j0 = 0; j0 < NB; j0 += PU) {
 #pragma unroll
   for (int u = 0; u < PU; ++u) {
@@ -237,11 +293,12 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     // reciprocal of the value, formed by every lane before the wave's winner is
     // known, so that the division overlaps the reductions (only the winning
     // lane's are published).
-    T pv = v[0][u], pnx = v[0][u + 1];
+    T pv = v[0][u], pnx = v[0][u + 1], gpv = ONE_BARRIER ? g[0] : T(0);
 #pragma unroll
     for (int q = 1; q < RPT; ++q) {
       pv = (cq == q) ? v[q][u] : pv;
       pnx = (cq == q) ? v[q][u + 1] : pnx;
+      if constexpr (ONE_BARRIER) gpv = (cq == q) ? g[q] : gpv;
     }
     const T rinv = T(1) / pv;
     // Exact ordering key of the candidate: the bits of |a| plus one (monotone in
@@ -279,27 +336,30 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
       }
     }
     // The wave's winning lane publishes its candidate's record: key, position,
-    // row, the reciprocal of its pivot value and its value in the next column
-    // (two 16-byte writes; the row itself is written once the global winner is
-    // known).
+    // row, the reciprocal of its pivot value, its value in the next column and
+    // its multiplier of the previous pivot (the row itself is written once the
+    // global winner is known).
     if (lane == wl) {
-      PanelRec<T> rec;
+      PanelRec<T, ONE_BARRIER> rec;
       rec.key = key;
       rec.pos = cp;
       rec.row = t + THREADS * cq;
       rec.pinv = rinv;
       rec.pnext = pnx;
+      if constexpr (ONE_BARRIER) rec.gprev = gpv;
       s_rec[buf][wave] = rec;
     } else if (wl < 0 && lane == 0) {
-      PanelRec<T> rec;
+      PanelRec<T, ONE_BARRIER> rec;
       rec.key = 0ull;
       rec.pos = 0x7fffffff;
       rec.row = 0;
       rec.pinv = T(0);
       rec.pnext = T(0);
+      if constexpr (ONE_BARRIER) rec.gprev = T(0);
       s_rec[buf][wave] = rec;
     }
     __syncthreads();
+    if (ONE_BARRIER && j > 0) prev_update(u, s_prow[buf ^ 1]);
     // Global winner: larger key, smaller position on ties.  Lane l reads wave
     // (l mod NW)'s record, so every NW consecutive lanes of a 16-lane row hold
     // all records, and log2(NW) DPP row rotations leave the winner in every
@@ -307,10 +367,12 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     // bw, which loaded wave bw's record.  (A wave without a candidate has key 0
     // and never wins: some wave always has a candidate.)
     const int wr = lane & (NW - 1);
-    const PanelRec<T> rr = s_rec[buf][wr];
+    const PanelRec<T, ONE_BARRIER> rr = s_rec[buf][wr];
     const int rrow = rr.row;
     const T rpinv = rr.pinv;
     const T rpn = rr.pnext;
+    T rgp = T(0);
+    if constexpr (ONE_BARRIER) rgp = rr.gprev;
     // First the top 32 bits of the keys with their lowest log2(NW) bits
     // replaced by the wave (smaller wave ranks higher): when only one wave's key
     // reaches the maximum there, that wave's full key is larger than every
@@ -339,9 +401,13 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     }
     const int pr = __builtin_amdgcn_readlane(rrow, bw);
     const T inv = readlane_t<T>(rpinv, bw);
-    const T pn = readlane_t<T>(rpn, bw);
-    // The pivot row's owner writes it for every thread; the second barrier
-    // below, after the next column's update, publishes it.
+    T pn = readlane_t<T>(rpn, bw);
+    // (the winner's next-column value through pivot j - 1, as its own row
+    // update formed it)
+    if (ONE_BARRIER && j > 0)
+      pn = E::fma(-readlane_t<T>(rgp, bw), s_prow[buf ^ 1][(u + 1 + (u == 0 ? PU : 0)) % NB], pn);
+    // The pivot row's owner writes it for every thread; the next column's
+    // barrier (or the second barrier of this one) publishes it.
     if (t == (pr & (THREADS - 1))) {
       const int pq = pr / THREADS;
 #pragma unroll
@@ -360,7 +426,7 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     // pivot row's own update is a no-op through g = 0, so no lane branches.
     // The next column is updated first: the next pivot search needs only it.
     if (t == 0) s_inv[j] = inv;
-    T g[RPT];
+    ppr = pr;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
       const bool is_piv = (t + THREADS * q == pr);
@@ -372,17 +438,10 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     // the window; pn is the pivot row's value there either way)
 #pragma unroll
     for (int q = 0; q < RPT; ++q) v[q][u + 1] = E::fma(-g[q], pn, v[q][u + 1]);
-    __syncthreads();
-    const T* prow = s_prow[buf];
-#pragma unroll
-    for (int l = 0; l < NB; ++l) {
-      if (l == u || l == u + 1) continue;
-      const T pl = prow[l];
-#pragma unroll
-      for (int q = 0; q < RPT; ++q) v[q][l] = E::fma(-g[q], pl, v[q][l]);
+    if constexpr (!ONE_BARRIER) {
+      __syncthreads();
+      cur_update(u, s_prow[buf], pr);
     }
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) v[q][u] = (t + THREADS * q == pr) ? T(1) : -g[q];
     }
     // rotate the window by PU columns
 #pragma unroll
@@ -397,8 +456,11 @@ __device__ __forceinline__ void gj_panel_body(const T* __restrict__ X, long long
     }
   }
 
-  // Deferred pivot-row scaling of this panel's pivot rows.
+  // The last pivot's update of the other columns (one-barrier schedule; the
+  // window is back at column 0), then the deferred pivot-row scaling of this
+  // panel's pivot rows.
   __syncthreads();
+  if constexpr (ONE_BARRIER) prev_update(0, s_prow[(NB - 1) & 1]);
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     if (pos[q] >= k0 && pos[q] < k0 + NB) {
